@@ -1,0 +1,427 @@
+// capi.cpp -- extern "C" boundary of the assembly engine (declared in
+// include/bempp_amd.h). Mirrors src/bindings/boundary_assemblers.rs and
+// src/bindings/matrices.rs of the reference: opaque boxed handles, caller-owned
+// output memory. Errors become a non-zero status / NULL plus bempp_last_error()
+// where the reference panics.
+#include <algorithm>
+#include <complex>
+#include <cstring>
+#include <numeric>
+
+#include "../../include/bempp_amd.h"
+#include "core.hpp"
+
+using namespace bempp_amd;
+
+struct BoundaryAssemblerOptionsT {
+  Options o;
+};
+struct BoundaryAssemblerT {
+  Assembler* a;
+};
+struct GridT {
+  Grid* g;
+};
+struct SpaceT {
+  FunctionSpace* s;
+};
+struct MatrixT {
+  bool csr = false;
+  DType dtype = kF64;
+  size_t shape0 = 0, shape1 = 0;
+  std::vector<unsigned char> data;  // dense column-major, or CSR values
+  std::vector<uintptr_t> indices, indptr;
+};
+
+namespace {
+
+template <typename F>
+auto guarded(F&& f, decltype(f()) fail) -> decltype(f()) {
+  try {
+    return f();
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+  } catch (...) {
+    set_last_error("unknown error");
+  }
+  return fail;
+}
+
+void need(const void* p, const char* what) {
+  if (!p) throw Error(std::string(what) + " is null");
+}
+
+// COO -> CSR with duplicates summed, rows then columns sorted (rlst
+// CsrMatrix::from_aij semantics, src/boundary_assemblers.rs:147-153); empty
+// when every |entry| <= 1e-10 (the hack at :130-145).
+template <typename T>
+void build_csr(MatrixT* m, const std::vector<int64_t>& rows, const std::vector<int64_t>& cols, const T* vals) {
+  const size_t n = rows.size();
+  m->indptr.assign(m->shape0 + 1, 0);
+  bool any = false;
+  for (size_t i = 0; i < n; ++i)
+    if (std::abs(vals[i]) > 1e-10) { any = true; break; }
+  if (n == 0 || !any) return;
+  std::vector<size_t> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+    return rows[a] != rows[b] ? rows[a] < rows[b] : cols[a] < cols[b];
+  });
+  std::vector<T> out;
+  for (size_t k = 0; k < n; ++k) {
+    const size_t i = order[k];
+    if (!m->indices.empty() && k > 0 && rows[order[k - 1]] == rows[i] && cols[order[k - 1]] == cols[i]) {
+      out.back() += vals[i];
+    } else {
+      m->indices.push_back((uintptr_t)cols[i]);
+      out.push_back(vals[i]);
+      m->indptr[rows[i] + 1]++;
+    }
+  }
+  for (size_t r = 0; r < m->shape0; ++r) m->indptr[r + 1] += m->indptr[r];
+  m->data.resize(out.size() * sizeof(T));
+  std::memcpy(m->data.data(), out.data(), m->data.size());
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* bempp_last_error(void) { return last_error(); }
+const char* bempp_version(void) { return "bempp-amd 0.1.0 (gfx950)"; }
+int bempp_device_count(void) { return device_count(); }
+
+// ---- options ----
+BoundaryAssemblerOptionsT* boundary_assembler_options_new(void) { return new BoundaryAssemblerOptionsT(); }
+void boundary_assembler_options_t_free(BoundaryAssemblerOptionsT* o) { delete o; }
+
+void boundary_assembler_options_set_regular_quadrature_degree(BoundaryAssemblerOptionsT* o, uint8_t cell,
+                                                               size_t npoints) {
+  if (!o) return;
+  auto it = o->o.quadrature_degrees.find(cell);
+  if (it != o->o.quadrature_degrees.end()) it->second = npoints;
+}
+size_t boundary_assembler_options_get_regular_quadrature_degree(const BoundaryAssemblerOptionsT* o, uint8_t cell) {
+  if (!o) return 0;
+  auto it = o->o.quadrature_degrees.find(cell);
+  return it == o->o.quadrature_degrees.end() ? 0 : it->second;
+}
+int boundary_assembler_options_has_regular_quadrature_degree(const BoundaryAssemblerOptionsT* o, uint8_t cell) {
+  return o && o->o.quadrature_degrees.count(cell) ? 1 : 0;
+}
+void boundary_assembler_options_set_singular_quadrature_degree(BoundaryAssemblerOptionsT* o, uint8_t c0, uint8_t c1,
+                                                                size_t npoints) {
+  if (!o) return;
+  auto it = o->o.singular_quadrature_degrees.find({c0, c1});
+  if (it != o->o.singular_quadrature_degrees.end()) it->second = npoints;
+}
+size_t boundary_assembler_options_get_singular_quadrature_degree(const BoundaryAssemblerOptionsT* o, uint8_t c0,
+                                                                 uint8_t c1) {
+  if (!o) return 0;
+  auto it = o->o.singular_quadrature_degrees.find({c0, c1});
+  return it == o->o.singular_quadrature_degrees.end() ? 0 : it->second;
+}
+int boundary_assembler_options_has_singular_quadrature_degree(const BoundaryAssemblerOptionsT* o, uint8_t c0,
+                                                              uint8_t c1) {
+  return o && o->o.singular_quadrature_degrees.count({c0, c1}) ? 1 : 0;
+}
+void boundary_assembler_options_set_batch_size(BoundaryAssemblerOptionsT* o, size_t b) {
+  if (o) o->o.batch_size = b;
+}
+size_t boundary_assembler_options_get_batch_size(const BoundaryAssemblerOptionsT* o) {
+  return o ? o->o.batch_size : 0;
+}
+int boundary_assembler_options_set_precision(BoundaryAssemblerOptionsT* o, uint8_t p) {
+  return guarded(
+      [&]() {
+        need(o, "options");
+        if (p != kFP64 && p != kMixed) throw Error("precision must be 0 (fp64) or 1 (mixed)");
+        o->o.precision = (Precision)p;
+        return 0;
+      },
+      1);
+}
+uint8_t boundary_assembler_options_get_precision(const BoundaryAssemblerOptionsT* o) {
+  return o ? (uint8_t)o->o.precision : 0;
+}
+
+// ---- grids ----
+GridT* grid_new(const double* points, size_t npoints, const int64_t* cells, size_t ncells) {
+  return guarded(
+      [&]() -> GridT* {
+        need(points, "points");
+        need(cells, "cells");
+        std::vector<double> p(points, points + 3 * npoints);
+        std::vector<int64_t> c(cells, cells + 3 * ncells);
+        return new GridT{new Grid(std::move(p), std::move(c))};
+      },
+      (GridT*)nullptr);
+}
+GridT* grid_new_sphere(int seed, uint32_t level) {
+  return guarded([&]() -> GridT* { return new GridT{make_sphere(seed, level)}; }, (GridT*)nullptr);
+}
+void grid_t_free(GridT* g) {
+  if (!g) return;
+  delete g->g;
+  delete g;
+}
+size_t grid_npoints(const GridT* g) { return g ? (size_t)g->g->nv() : 0; }
+size_t grid_ncells(const GridT* g) { return g ? (size_t)g->g->nc() : 0; }
+void grid_points_copy(const GridT* g, double* p) {
+  if (g && p) std::copy(g->g->points.begin(), g->g->points.end(), p);
+}
+void grid_cells_copy(const GridT* g, int64_t* c) {
+  if (g && c) std::copy(g->g->cells.begin(), g->g->cells.end(), c);
+}
+int grid_set_points(GridT* g, const double* p) {
+  return guarded(
+      [&]() {
+        need(g, "grid");
+        need(p, "points");
+        g->g->set_points(p);
+        return 0;
+      },
+      1);
+}
+
+// ---- spaces ----
+SpaceT* function_space_new(const GridT* g, size_t degree, uint8_t continuity) {
+  return guarded(
+      [&]() -> SpaceT* {
+        need(g, "grid");
+        if (continuity > 1) throw Error("continuity must be 0 (standard) or 1 (discontinuous)");
+        return new SpaceT{new FunctionSpace(g->g, (int)degree, continuity == BEMPP_CONTINUITY_STANDARD)};
+      },
+      (SpaceT*)nullptr);
+}
+void space_t_free(SpaceT* s) {
+  if (!s) return;
+  delete s->s;
+  delete s;
+}
+size_t space_global_size(const SpaceT* s) { return s ? (size_t)s->s->size : 0; }
+size_t space_local_size(const SpaceT* s) { return s ? (size_t)s->s->size : 0; }
+int space_is_serial(const SpaceT* s) { return s ? 1 : 0; }
+int space_has_cell_dofs(const SpaceT* s, size_t cell) { return s && cell < (size_t)s->s->grid->nc() ? 1 : 0; }
+size_t space_cell_dofs_size(const SpaceT* s, size_t cell) {
+  return space_has_cell_dofs(s, cell) ? (size_t)s->s->nb : 0;
+}
+void space_cell_dofs(const SpaceT* s, size_t cell, uintptr_t* dofs) {
+  if (!space_has_cell_dofs(s, cell) || !dofs) return;
+  for (int b = 0; b < s->s->nb; ++b) dofs[b] = (uintptr_t)s->s->dofs[cell * s->s->nb + b];
+}
+size_t space_global_dof_index(const SpaceT*, size_t local) { return local; }
+size_t space_cell_colouring(const SpaceT* s, int64_t* colour) {
+  if (!s) return 0;
+  auto cols = s->s->colouring();
+  if (colour)
+    for (size_t c = 0; c < cols.size(); ++c)
+      for (int64_t cell : cols[c]) colour[cell] = (int64_t)c;
+  return cols.size();
+}
+
+// ---- assemblers ----
+BoundaryAssemblerT* laplace_boundary_assembler_new(const BoundaryAssemblerOptionsT* o, uint8_t op, uint8_t dtype) {
+  return guarded(
+      [&]() -> BoundaryAssemblerT* {
+        need(o, "options");
+        if (op > kHS) throw Error("Invalid operator for Laplace: " + std::to_string(op));
+        if (dtype != kF32 && dtype != kF64) throw Error("Invalid dtype for Laplace: " + std::to_string(dtype));
+        auto* a = new Assembler();
+        a->kind = kLaplace;
+        a->op = (Op)op;
+        a->dtype = (DType)dtype;
+        a->options = o->o;
+        return new BoundaryAssemblerT{a};
+      },
+      (BoundaryAssemblerT*)nullptr);
+}
+BoundaryAssemblerT* helmholtz_boundary_assembler_new(const void* k, const BoundaryAssemblerOptionsT* o, uint8_t op,
+                                                     uint8_t dtype) {
+  return guarded(
+      [&]() -> BoundaryAssemblerT* {
+        need(o, "options");
+        need(k, "wavenumber");
+        if (op > kHS) throw Error("Invalid operator for Helmholtz: " + std::to_string(op));
+        if (dtype != kC32 && dtype != kC64) throw Error("Invalid dtype for Helmholtz: " + std::to_string(dtype));
+        auto* a = new Assembler();
+        a->kind = kHelmholtz;
+        a->op = (Op)op;
+        a->dtype = (DType)dtype;
+        a->k = dtype == kC32 ? (double)*(const float*)k : *(const double*)k;
+        a->options = o->o;
+        return new BoundaryAssemblerT{a};
+      },
+      (BoundaryAssemblerT*)nullptr);
+}
+void boundary_assembler_t_free(BoundaryAssemblerT* a) {
+  if (!a) return;
+  delete a->a;
+  delete a;
+}
+uint8_t boundary_assembler_dtype(const BoundaryAssemblerT* a) { return a ? (uint8_t)a->a->dtype : 255; }
+
+static void check_spaces(const BoundaryAssemblerT* a, const SpaceT* trial, const SpaceT* test) {
+  need(a, "assembler");
+  need(trial, "trial space");
+  need(test, "test space");
+}
+
+int boundary_assembler_assemble_into_memory(const BoundaryAssemblerT* a, const SpaceT* trial, const SpaceT* test,
+                                            void* data) {
+  return guarded(
+      [&]() {
+        check_spaces(a, trial, test);
+        assemble_host(*a->a, *trial->s, *test->s, data);
+        return 0;
+      },
+      1);
+}
+
+int boundary_assembler_assemble_into_device_memory(const BoundaryAssemblerT* a, const SpaceT* trial,
+                                                   const SpaceT* test, void* data, int accumulate, void* stream) {
+  return guarded(
+      [&]() {
+        check_spaces(a, trial, test);
+        assemble_device(*a->a, *trial->s, *test->s, data, accumulate != 0, stream);
+        return 0;
+      },
+      1);
+}
+
+MatrixT* boundary_assembler_assemble(const BoundaryAssemblerT* a, const SpaceT* trial, const SpaceT* test) {
+  return guarded(
+      [&]() -> MatrixT* {
+        check_spaces(a, trial, test);
+        auto* m = new MatrixT();
+        m->dtype = a->a->dtype;
+        m->shape0 = (size_t)test->s->size;
+        m->shape1 = (size_t)trial->s->size;
+        m->data.assign(m->shape0 * m->shape1 * dtype_size(m->dtype), 0);
+        try {
+          assemble_host(*a->a, *trial->s, *test->s, m->data.data());
+        } catch (...) {
+          delete m;
+          throw;
+        }
+        return m;
+      },
+      (MatrixT*)nullptr);
+}
+
+MatrixT* boundary_assembler_assemble_singular(const BoundaryAssemblerT* a, const SpaceT* trial, const SpaceT* test) {
+  return guarded(
+      [&]() -> MatrixT* {
+        check_spaces(a, trial, test);
+        std::vector<int64_t> rows, cols;
+        std::vector<double> vals;
+        assemble_singular_coo(*a->a, *trial->s, *test->s, rows, cols, vals);
+        auto* m = new MatrixT();
+        m->csr = true;
+        m->dtype = a->a->dtype;
+        m->shape0 = (size_t)test->s->size;
+        m->shape1 = (size_t)trial->s->size;
+        switch (m->dtype) {
+          case kF64: build_csr<double>(m, rows, cols, vals.data()); break;
+          case kC64: build_csr<std::complex<double>>(m, rows, cols, (const std::complex<double>*)vals.data()); break;
+          case kF32: {
+            std::vector<float> v(vals.begin(), vals.end());
+            build_csr<float>(m, rows, cols, v.data());
+            break;
+          }
+          default: {
+            std::vector<std::complex<float>> v(vals.size() / 2);
+            for (size_t i = 0; i < v.size(); ++i) v[i] = {(float)vals[2 * i], (float)vals[2 * i + 1]};
+            build_csr<std::complex<float>>(m, rows, cols, v.data());
+            break;
+          }
+        }
+        return m;
+      },
+      (MatrixT*)nullptr);
+}
+
+int boundary_assembler_shape_gradient(const BoundaryAssemblerT* a, const SpaceT* trial, const SpaceT* test,
+                                      const void* u, const void* v, void* grad) {
+  return guarded(
+      [&]() {
+        check_spaces(a, trial, test);
+        need(u, "u");
+        need(v, "v");
+        need(grad, "grad");
+        shape_gradient(*a->a, *trial->s, *test->s, u, v, grad);
+        return 0;
+      },
+      1);
+}
+
+int boundary_assembler_set_profiling(BoundaryAssemblerT* a, int enable) {
+  return guarded(
+      [&]() {
+        need(a, "assembler");
+        a->a->profiling = enable != 0;
+        return 0;
+      },
+      1);
+}
+
+int boundary_assembler_kernel_times(const BoundaryAssemblerT* a, float* ms) {
+  return guarded(
+      [&]() {
+        need(a, "assembler");
+        need(ms, "ms");
+        kernel_times(*a->a, ms);
+        return 0;
+      },
+      1);
+}
+
+// ---- matrices ----
+size_t matrix_shape0(const MatrixT* m) { return m ? m->shape0 : 0; }
+size_t matrix_shape1(const MatrixT* m) { return m ? m->shape1 : 0; }
+int matrix_is_csr(const MatrixT* m) { return m && m->csr ? 1 : 0; }
+uint8_t matrix_dtype(const MatrixT* m) { return m ? (uint8_t)m->dtype : 255; }
+const void* matrix_dense_data(const MatrixT* m) { return m && !m->csr ? m->data.data() : nullptr; }
+void matrix_dense_data_copy(const MatrixT* m, void* d) {
+  if (m && !m->csr && d) std::memcpy(d, m->data.data(), m->data.size());
+}
+size_t matrix_csr_data_size(const MatrixT* m) { return m && m->csr ? m->data.size() / dtype_size(m->dtype) : 0; }
+size_t matrix_csr_indices_size(const MatrixT* m) { return m && m->csr ? m->indices.size() : 0; }
+size_t matrix_csr_indptr_size(const MatrixT* m) { return m && m->csr ? m->indptr.size() : 0; }
+void matrix_csr_data_copy(const MatrixT* m, void* d) {
+  if (m && m->csr && d) std::memcpy(d, m->data.data(), m->data.size());
+}
+void matrix_csr_indices_copy(const MatrixT* m, uintptr_t* d) {
+  if (m && m->csr && d) std::copy(m->indices.begin(), m->indices.end(), d);
+}
+void matrix_csr_indptr_copy(const MatrixT* m, uintptr_t* d) {
+  if (m && m->csr && d) std::copy(m->indptr.begin(), m->indptr.end(), d);
+}
+void matrix_t_free(MatrixT* m) { delete m; }
+
+// Quadrature tables used by the engine (no counterpart in the reference; they
+// expose what ndelement::quadrature::simplex_rule and
+// bempp_quadrature::duffy::triangle_duffy provide to the reference).
+int bempp_triangle_rule(int npoints, double* xi, double* w) {
+  return guarded(
+      [&]() {
+        std::vector<double> a, b;
+        triangle_rule(npoints, a, b);
+        if (xi) std::copy(a.begin(), a.end(), xi);
+        if (w) std::copy(b.begin(), b.end(), w);
+        return (int)b.size();
+      },
+      -1);
+}
+int bempp_canonical_pair_rule(int nshared, int n, double* pts5) {
+  return guarded(
+      [&]() {
+        std::vector<double> r;
+        canonical_pair_rule(nshared, n, r);
+        if (pts5) std::copy(r.begin(), r.end(), pts5);
+        return (int)(r.size() / 5);
+      },
+      -1);
+}
+
+}  // extern "C"

@@ -1,0 +1,553 @@
+// device.hip -- device-side runtime of the assembly engine: per-grid geometry
+// caches in HBM, rule tables, kernel launches, and the host entry points
+// (dense into device or host memory, singular COO).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+
+#include "core.hpp"
+#include "kernels.hpp"
+
+namespace bempp_amd {
+
+#define HIP_CHECK(expr)                                                                           \
+  do {                                                                                            \
+    hipError_t _e = (expr);                                                                       \
+    if (_e != hipSuccess) throw Error(std::string(#expr) + " failed: " + hipGetErrorString(_e)); \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Device caches
+// ---------------------------------------------------------------------------
+struct DeviceGridCache {
+  int device = -1;
+  double* geo = nullptr;          // [nc][kGeoStride]
+  int* vid = nullptr;             // [nc][4]
+  std::map<int, double*> pts;     // regular point count -> [nc][Q][4]
+  int4* pairs = nullptr;          // adjacent pairs
+  int npairs = -1;
+  std::vector<AdjacentPair> host_pairs;
+};
+
+struct DeviceSpaceCache {
+  int device = -1;
+  int* dofs = nullptr;  // [nc][nb] int32
+};
+
+void free_device_grid_cache(DeviceGridCache* c) {
+  if (!c) return;
+  int cur = 0;
+  if (hipGetDevice(&cur) == hipSuccess && hipSetDevice(c->device) == hipSuccess) {
+    (void)hipFree(c->geo);
+    (void)hipFree(c->vid);
+    for (auto& kv : c->pts) (void)hipFree(kv.second);
+    (void)hipFree(c->pairs);
+    (void)hipSetDevice(cur);
+  }
+  delete c;
+}
+
+void free_device_space_cache(DeviceSpaceCache* c) {
+  if (!c) return;
+  int cur = 0;
+  if (hipGetDevice(&cur) == hipSuccess && hipSetDevice(c->device) == hipSuccess) {
+    (void)hipFree(c->dofs);
+    (void)hipSetDevice(cur);
+  }
+  delete c;
+}
+
+int device_count() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+namespace {
+
+int current_device() {
+  int n = device_count();
+  if (n <= 0) throw Error("no HIP device available: assembly requires an MI355X GPU (there is no CPU fallback)");
+  int d = 0;
+  HIP_CHECK(hipGetDevice(&d));
+  return d;
+}
+
+template <typename T>
+T* upload(const std::vector<T>& h) {
+  T* d = nullptr;
+  HIP_CHECK(hipMalloc((void**)&d, std::max<size_t>(h.size(), 1) * sizeof(T)));
+  if (!h.empty()) HIP_CHECK(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return d;
+}
+
+// Per-cell geometry record: vertices, unit normal, jdet = |J0 x J1|, and
+// curlJ_b = J1 d0(phi_b) - J0 d1(phi_b) for the P1 basis (constant on a flat cell).
+void cell_record(const Grid& g, int64_t c, double* r) {
+  const double* v[3];
+  for (int m = 0; m < 3; ++m) v[m] = &g.points[3 * g.cells[3 * c + m]];
+  double J0[3], J1[3];
+  for (int d = 0; d < 3; ++d) {
+    r[d] = v[0][d];
+    r[3 + d] = v[1][d];
+    r[6 + d] = v[2][d];
+    J0[d] = v[1][d] - v[0][d];
+    J1[d] = v[2][d] - v[0][d];
+  }
+  const double n[3] = {J0[1] * J1[2] - J0[2] * J1[1], J0[2] * J1[0] - J0[0] * J1[2], J0[0] * J1[1] - J0[1] * J1[0]};
+  const double jdet = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  if (!(jdet > 0.0)) throw Error("cell " + std::to_string(c) + " is degenerate (zero area)");
+  for (int d = 0; d < 3; ++d) {
+    r[9 + d] = n[d] / jdet;
+    r[13 + d] = J0[d] - J1[d];  // basis 0: grad (-1,-1)
+    r[16 + d] = J1[d];          // basis 1: grad (1,0)
+    r[19 + d] = -J0[d];         // basis 2: grad (0,1)
+  }
+  r[12] = jdet;
+  r[22] = r[23] = 0.0;
+}
+
+DeviceGridCache* grid_cache(Grid& g, int device) {
+  std::lock_guard<std::mutex> lk(g.cache_mutex);
+  auto it = g.device_cache.find(device);
+  if (it != g.device_cache.end()) return it->second;
+  auto* c = new DeviceGridCache();
+  c->device = device;
+  const int64_t nc = g.nc();
+  std::vector<double> geo(nc * dev::kGeoStride);
+  std::vector<int> vid(nc * 4);
+  for (int64_t i = 0; i < nc; ++i) {
+    cell_record(g, i, &geo[i * dev::kGeoStride]);
+    for (int m = 0; m < 3; ++m) vid[4 * i + m] = (int)g.cells[3 * i + m];
+    vid[4 * i + 3] = -1;
+  }
+  try {
+    c->geo = upload(geo);
+    c->vid = upload(vid);
+  } catch (...) {
+    free_device_grid_cache(c);
+    throw;
+  }
+  g.device_cache[device] = c;
+  return c;
+}
+
+const double* grid_points_for_rule(Grid& g, DeviceGridCache* c, int Q, const std::vector<double>& xi,
+                                   const std::vector<double>& w) {
+  std::lock_guard<std::mutex> lk(g.cache_mutex);
+  auto it = c->pts.find(Q);
+  if (it != c->pts.end()) return it->second;
+  const int64_t nc = g.nc();
+  std::vector<double> pts(nc * Q * 4);
+  for (int64_t i = 0; i < nc; ++i) {
+    const double* v0 = &g.points[3 * g.cells[3 * i]];
+    const double* v1 = &g.points[3 * g.cells[3 * i + 1]];
+    const double* v2 = &g.points[3 * g.cells[3 * i + 2]];
+    double J0[3], J1[3];
+    for (int d = 0; d < 3; ++d) {
+      J0[d] = v1[d] - v0[d];
+      J1[d] = v2[d] - v0[d];
+    }
+    const double cr[3] = {J0[1] * J1[2] - J0[2] * J1[1], J0[2] * J1[0] - J0[0] * J1[2],
+                          J0[0] * J1[1] - J0[1] * J1[0]};
+    const double jdet = std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+    for (int p = 0; p < Q; ++p) {
+      double* o = &pts[(i * Q + p) * 4];
+      for (int d = 0; d < 3; ++d) o[d] = v0[d] + J0[d] * xi[2 * p] + J1[d] * xi[2 * p + 1];
+      o[3] = w[p] * jdet;
+    }
+  }
+  double* d = upload(pts);
+  c->pts[Q] = d;
+  return d;
+}
+
+const int4* grid_pairs(Grid& g, DeviceGridCache* c, int* npairs) {
+  std::lock_guard<std::mutex> lk(g.cache_mutex);
+  if (c->npairs < 0) {
+    adjacent_pairs(g, c->host_pairs);
+    std::vector<int4> p(c->host_pairs.size());
+    for (size_t i = 0; i < p.size(); ++i) {
+      const AdjacentPair& a = c->host_pairs[i];
+      p[i] = make_int4(a.test, a.trial, a.nshared, a.perm);
+    }
+    c->pairs = upload(p);
+    c->npairs = (int)p.size();
+  }
+  *npairs = c->npairs;
+  return c->pairs;
+}
+
+const int* space_dofs(FunctionSpace& s, int device) {
+  std::lock_guard<std::mutex> lk(s.cache_mutex);
+  auto it = s.device_cache.find(device);
+  if (it != s.device_cache.end()) return it->second->dofs;
+  if (s.size > INT32_MAX) throw Error("function space too large for 32-bit DOF indices");
+  std::vector<int> d(s.dofs.begin(), s.dofs.end());
+  auto* c = new DeviceSpaceCache();
+  c->device = device;
+  c->dofs = upload(d);
+  s.device_cache[device] = c;
+  return c->dofs;
+}
+
+// Canonical Sauter-Schwab rules for one Gauss order n, per device (kept for the
+// life of the process: a few hundred KB).
+struct RuleTable {
+  double* d = nullptr;
+  int total = 0;
+  int off[4] = {0, 0, 0, 0};
+  int n[4] = {0, 0, 0, 0};
+};
+
+const RuleTable& rule_table(int device, int n) {
+  static std::mutex mu;
+  static std::map<std::pair<int, int>, RuleTable> tables;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = tables.find({device, n});
+  if (it != tables.end()) return it->second;
+  RuleTable t;
+  std::vector<double> parts[4];
+  for (int m = 1; m <= 3; ++m) {
+    canonical_pair_rule(m, n, parts[m]);
+    t.off[m] = t.total;
+    t.n[m] = (int)(parts[m].size() / 5);
+    t.total += t.n[m];
+  }
+  std::vector<double> soa(5 * (size_t)t.total);
+  for (int m = 1; m <= 3; ++m)
+    for (int i = 0; i < t.n[m]; ++i)
+      for (int f = 0; f < 5; ++f) soa[(size_t)f * t.total + t.off[m] + i] = parts[m][5 * i + f];
+  t.d = upload(soa);
+  return tables.emplace(std::make_pair(device, n), t).first->second;
+}
+
+hipStream_t assembler_stream(const Assembler& a, int device) {
+  auto& m = const_cast<Assembler&>(a);
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!m.stream || m.device != device) {
+    if (m.stream) {
+      int cur = 0;
+      (void)hipGetDevice(&cur);
+      (void)hipSetDevice(m.device);
+      (void)hipStreamDestroy((hipStream_t)m.stream);
+      (void)hipSetDevice(cur);
+    }
+    hipStream_t s;
+    HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    m.stream = s;
+    m.device = device;
+  }
+  return (hipStream_t)m.stream;
+}
+
+int out_type(DType d) {
+  switch (d) {
+    case kF64: return dev::kOutF64;
+    case kC64: return dev::kOutC128;
+    case kF32: return dev::kOutF32;
+    default: return dev::kOutC64;
+  }
+}
+
+int regular_points(const Options& o) {
+  auto it = o.quadrature_degrees.find(kTriangle);
+  if (it == o.quadrature_degrees.end()) throw Error("no regular quadrature degree set for triangles");
+  return (int)it->second;
+}
+
+int singular_points(const Options& o) {
+  auto it = o.singular_quadrature_degrees.find({kTriangle, kTriangle});
+  if (it == o.singular_quadrature_degrees.end()) throw Error("no singular quadrature degree set for triangle pairs");
+  return (int)it->second;
+}
+
+// ---- kernel dispatch tables ----
+template <int OP, bool HELM, bool P0P0>
+void launch_regular_q(int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& A) {
+  switch (Q) {
+    case 1: dev::regular_kernel<1, OP, HELM, P0P0><<<grid, 256, 0, s>>>(A); break;
+    case 3: dev::regular_kernel<3, OP, HELM, P0P0><<<grid, 256, 0, s>>>(A); break;
+    case 6: dev::regular_kernel<6, OP, HELM, P0P0><<<grid, 256, 0, s>>>(A); break;
+    case 7: dev::regular_kernel<7, OP, HELM, P0P0><<<grid, 256, 0, s>>>(A); break;
+    default: throw Error("unsupported regular quadrature point count");
+  }
+}
+
+template <bool HELM, bool P0P0>
+void launch_regular_op(int op, int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& A) {
+  switch (op) {
+    case kSL: launch_regular_q<kSL, HELM, P0P0>(Q, grid, s, A); break;
+    case kDL: launch_regular_q<kDL, HELM, P0P0>(Q, grid, s, A); break;
+    case kADL: launch_regular_q<kADL, HELM, P0P0>(Q, grid, s, A); break;
+    case kHS: launch_regular_q<kHS, HELM, P0P0>(Q, grid, s, A); break;
+    default: throw Error("operator not supported by the dense assembler");
+  }
+}
+
+void launch_regular(int op, bool helm, bool p0p0, int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& A) {
+  if (helm) {
+    if (p0p0) launch_regular_op<true, true>(op, Q, grid, s, A);
+    else launch_regular_op<true, false>(op, Q, grid, s, A);
+  } else {
+    if (p0p0) launch_regular_op<false, true>(op, Q, grid, s, A);
+    else launch_regular_op<false, false>(op, Q, grid, s, A);
+  }
+  HIP_CHECK(hipGetLastError());
+}
+
+template <bool HELM>
+void launch_singular_op(int op, dim3 grid, hipStream_t s, const dev::SingularArgs& A) {
+  switch (op) {
+    case kSL: dev::singular_kernel<kSL, HELM><<<grid, 256, 0, s>>>(A); break;
+    case kDL: dev::singular_kernel<kDL, HELM><<<grid, 256, 0, s>>>(A); break;
+    case kADL: dev::singular_kernel<kADL, HELM><<<grid, 256, 0, s>>>(A); break;
+    case kHS: dev::singular_kernel<kHS, HELM><<<grid, 256, 0, s>>>(A); break;
+    default: throw Error("operator not supported by the dense assembler");
+  }
+  HIP_CHECK(hipGetLastError());
+}
+
+struct Prepared {
+  int device;
+  hipStream_t stream;
+  int Q;
+  bool same_grid;
+  dev::RegularArgs R;
+  dev::SingularArgs S;
+  int npairs;
+};
+
+void check_assembler(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test) {
+  if (a.op != kSL && a.op != kDL && a.op != kADL && a.op != kHS)
+    throw Error("operator type not supported for dense assembly");
+  if ((a.kind == kLaplace) == is_complex(a.dtype)) throw Error("dtype does not match the kernel");
+  (void)trial;
+  (void)test;
+}
+
+Prepared prepare(const Assembler& a, const FunctionSpace& trial_c, const FunctionSpace& test_c, void* out,
+                 void* stream) {
+  check_assembler(a, trial_c, test_c);
+  auto& trial = const_cast<FunctionSpace&>(trial_c);
+  auto& test = const_cast<FunctionSpace&>(test_c);
+  auto& tgrid = const_cast<Grid&>(*test.grid);
+  auto& sgrid = const_cast<Grid&>(*trial.grid);
+  Prepared P{};
+  P.device = current_device();
+  P.stream = stream ? (hipStream_t)stream : assembler_stream(a, P.device);
+  P.Q = regular_points(a.options);
+  std::vector<double> xi, w;
+  triangle_rule(P.Q, xi, w);
+  const int nsing = singular_points(a.options);
+  if (nsing < 1 || nsing > 64) throw Error("singular quadrature degree must be in [1, 64]");
+  P.same_grid = test.grid == trial.grid;
+
+  DeviceGridCache* tc = grid_cache(tgrid, P.device);
+  DeviceGridCache* sc = P.same_grid ? tc : grid_cache(sgrid, P.device);
+  dev::RegularArgs& R = P.R;
+  R.test_pts = grid_points_for_rule(tgrid, tc, P.Q, xi, w);
+  R.trial_pts = grid_points_for_rule(sgrid, sc, P.Q, xi, w);
+  R.test_geo = tc->geo;
+  R.trial_geo = sc->geo;
+  R.test_vid = tc->vid;
+  R.trial_vid = sc->vid;
+  R.test_dofs = space_dofs(test, P.device);
+  R.trial_dofs = space_dofs(trial, P.device);
+  R.n_test = (int)tgrid.nc();
+  R.n_trial = (int)sgrid.nc();
+  R.nbt = test.nb;
+  R.nbs = trial.nb;
+  R.same_grid = P.same_grid ? 1 : 0;
+  R.k = a.k;
+  for (int p = 0; p < dev::kMaxQ; ++p)
+    for (int b = 0; b < 3; ++b) R.phi[p][b] = 0.0;
+  for (int p = 0; p < P.Q; ++p) {
+    R.phi[p][0] = 1.0 - xi[2 * p] - xi[2 * p + 1];
+    R.phi[p][1] = xi[2 * p];
+    R.phi[p][2] = xi[2 * p + 1];
+  }
+  R.out = out;
+  R.ld = (long long)test.size;
+  R.out_type = out_type(a.dtype);
+
+  P.npairs = 0;
+  if (P.same_grid) {
+    const RuleTable& rt = rule_table(P.device, nsing);
+    dev::SingularArgs& S = P.S;
+    S.pairs = grid_pairs(tgrid, tc, &P.npairs);
+    S.npairs = P.npairs;
+    S.rule = rt.d;
+    S.rule_total = rt.total;
+    for (int m = 0; m < 4; ++m) {
+      S.rule_off[m] = rt.off[m];
+      S.rule_n[m] = rt.n[m];
+    }
+    S.test_geo = tc->geo;
+    S.trial_geo = tc->geo;
+    S.test_dofs = R.test_dofs;
+    S.trial_dofs = R.trial_dofs;
+    S.nbt = test.nb;
+    S.nbs = trial.nb;
+    S.k = a.k;
+    S.out = out;
+    S.ld = R.ld;
+    S.out_type = R.out_type;
+  }
+  return P;
+}
+
+}  // namespace
+
+void assemble_device(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test, void* dev_out,
+                     bool accumulate, void* stream) {
+  if (!dev_out) throw Error("output pointer is null");
+  Prepared P = prepare(a, trial, test, dev_out, stream);
+  const bool p0p0 = test.nb == 1 && trial.nb == 1;
+  const long long n_out = (long long)test.size * (long long)trial.size * (is_complex(a.dtype) ? 2 : 1);
+  if (!p0p0 && !accumulate) {
+    if (a.dtype == kF32 || a.dtype == kC32) {
+      HIP_CHECK(hipMemsetAsync(dev_out, 0, (size_t)n_out * 4, P.stream));
+    } else {
+      dev::fill_zero_kernel<<<2048, 256, 0, P.stream>>>((double*)dev_out, n_out);
+      HIP_CHECK(hipGetLastError());
+    }
+  }
+  // regular pairs
+  dev::RegularArgs R = P.R;
+  R.mode = p0p0 ? (accumulate ? dev::kAccumulate : dev::kStore) : dev::kAtomic;
+  const int bx = (R.n_test + 255) / 256;
+  // trial chunk: enough blocks to fill 256 CUs several times over
+  long long chunk = ((long long)bx * R.n_trial + 16383) / 16384;
+  chunk = std::max(4LL, std::min(128LL, chunk));
+  chunk = std::max(chunk, ((long long)R.n_trial + 65534) / 65535);
+  R.trial_chunk = (int)chunk;
+  dim3 grid(bx, (unsigned)((R.n_trial + chunk - 1) / chunk));
+  auto& am = const_cast<Assembler&>(a);
+  if (a.profiling) {
+    for (int e = 0; e < 4; ++e)
+      if (!am.events[e]) {
+        hipEvent_t ev;
+        HIP_CHECK(hipEventCreate(&ev));
+        am.events[e] = ev;
+      }
+    am.events_recorded[0] = am.events_recorded[1] = false;
+  }
+  if (R.n_test > 0 && R.n_trial > 0) {
+    if (a.profiling) HIP_CHECK(hipEventRecord((hipEvent_t)am.events[0], P.stream));
+    launch_regular(a.op, a.kind == kHelmholtz, p0p0, P.Q, grid, P.stream, R);
+    if (a.profiling) {
+      HIP_CHECK(hipEventRecord((hipEvent_t)am.events[1], P.stream));
+      am.events_recorded[0] = true;
+    }
+  }
+  // adjacent pairs
+  if (P.npairs > 0) {
+    dev::SingularArgs S = P.S;
+    S.mode = p0p0 ? dev::kAccumulate : dev::kAtomic;
+    dim3 sg((unsigned)((P.npairs + 3) / 4));
+    if (a.profiling) HIP_CHECK(hipEventRecord((hipEvent_t)am.events[2], P.stream));
+    if (a.kind == kHelmholtz) launch_singular_op<true>(a.op, sg, P.stream, S);
+    else launch_singular_op<false>(a.op, sg, P.stream, S);
+    if (a.profiling) {
+      HIP_CHECK(hipEventRecord((hipEvent_t)am.events[3], P.stream));
+      am.events_recorded[1] = true;
+    }
+  }
+}
+
+void kernel_times(const Assembler& a, float* ms) {
+  for (int k = 0; k < 2; ++k) {
+    ms[k] = -1.0f;
+    if (!a.events_recorded[k]) continue;
+    HIP_CHECK(hipEventSynchronize((hipEvent_t)a.events[2 * k + 1]));
+    HIP_CHECK(hipEventElapsedTime(&ms[k], (hipEvent_t)a.events[2 * k], (hipEvent_t)a.events[2 * k + 1]));
+  }
+}
+
+void assemble_host(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test, void* host_out) {
+  if (!host_out) throw Error("output pointer is null");
+  const int device = current_device();
+  (void)device;
+  const size_t bytes = (size_t)test.size * (size_t)trial.size * dtype_size(a.dtype);
+  void* d = nullptr;
+  HIP_CHECK(hipMalloc(&d, std::max<size_t>(bytes, 1)));
+  try {
+    hipStream_t s = assembler_stream(a, device);
+    if (bytes) HIP_CHECK(hipMemcpyAsync(d, host_out, bytes, hipMemcpyHostToDevice, s));
+    assemble_device(a, trial, test, d, true, s);
+    if (bytes) HIP_CHECK(hipMemcpyAsync(host_out, d, bytes, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+  } catch (...) {
+    (void)hipFree(d);
+    throw;
+  }
+  HIP_CHECK(hipFree(d));
+}
+
+void assemble_singular_coo(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test,
+                           std::vector<int64_t>& rows, std::vector<int64_t>& cols, std::vector<double>& vals) {
+  rows.clear();
+  cols.clear();
+  vals.clear();
+  if (test.grid != trial.grid) return;  // no neighbouring cells across grids (:240-243)
+  Prepared P = prepare(a, trial, test, nullptr, nullptr);
+  if (P.npairs == 0) return;
+  const int nbl = test.nb * trial.nb;
+  const size_t nvals = (size_t)P.npairs * nbl;
+  double* d = nullptr;
+  HIP_CHECK(hipMalloc((void**)&d, nvals * 2 * sizeof(double)));
+  std::vector<double> h(nvals * 2);
+  try {
+    dev::SingularArgs S = P.S;
+    S.mode = dev::kCoo;
+    S.out = d;
+    dim3 sg((unsigned)((P.npairs + 3) / 4));
+    if (a.kind == kHelmholtz) launch_singular_op<true>(a.op, sg, P.stream, S);
+    else launch_singular_op<false>(a.op, sg, P.stream, S);
+    HIP_CHECK(hipMemcpyAsync(h.data(), d, h.size() * sizeof(double), hipMemcpyDeviceToHost, P.stream));
+    HIP_CHECK(hipStreamSynchronize(P.stream));
+  } catch (...) {
+    (void)hipFree(d);
+    throw;
+  }
+  HIP_CHECK(hipFree(d));
+  const auto& pairs = const_cast<Grid&>(*test.grid).device_cache.at(P.device)->host_pairs;
+  rows.resize(nvals);
+  cols.resize(nvals);
+  const bool cplx = is_complex(a.dtype);
+  vals.resize(nvals * (cplx ? 2 : 1));
+  for (size_t p = 0; p < pairs.size(); ++p)
+    for (int b = 0; b < trial.nb; ++b)
+      for (int t = 0; t < test.nb; ++t) {
+        const size_t e = p * nbl + t + test.nb * b;
+        rows[e] = test.dofs[(size_t)pairs[p].test * test.nb + t];
+        cols[e] = trial.dofs[(size_t)pairs[p].trial * trial.nb + b];
+        if (cplx) {
+          vals[2 * e] = h[2 * e];
+          vals[2 * e + 1] = h[2 * e + 1];
+        } else {
+          vals[e] = h[2 * e];
+        }
+      }
+}
+
+Assembler::~Assembler() {
+  for (void* e : events)
+    if (e) (void)hipEventDestroy((hipEvent_t)e);
+  if (stream) {
+    int cur = 0;
+    if (hipGetDevice(&cur) == hipSuccess) {
+      (void)hipSetDevice(device);
+      (void)hipStreamDestroy((hipStream_t)stream);
+      (void)hipSetDevice(cur);
+    }
+  }
+}
+
+}  // namespace bempp_amd

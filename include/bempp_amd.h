@@ -1,0 +1,226 @@
+/* bempp_amd.h -- C ABI of the MI355X boundary-element assembly engine.
+ *
+ * The entry points mirror the reference's C bindings for dense and singular
+ * boundary-operator assembly (src/bindings/boundary_assemblers.rs,
+ * src/bindings/matrices.rs; those bindings are disabled in the reference,
+ * src/lib.rs:5, and the shape is kept). Each declaration cites the reference
+ * function it replaces. Additions with no counterpart in the reference say so.
+ *
+ * Conventions
+ *  - Handles are opaque and owned by the caller; every *_new/_create has a
+ *    matching *_t_free that accepts NULL.
+ *  - Dense matrices are column-major (Fortran order): entry (test dof i,
+ *    trial dof j) lives at data[i + n_test * j]; the trial space is passed
+ *    before the test space, as in the reference (src/boundary_assemblers.rs:158-214).
+ *  - assemble_into_memory ACCUMULATES (+=) into the caller's buffer
+ *    (src/boundary_assemblers.rs:176-214).
+ *  - Where the reference panics, these functions return a non-zero status
+ *    (or NULL) and set a message readable with bempp_last_error().
+ *  - Assembly runs on the current HIP device; there is no CPU fallback: with
+ *    no usable GPU the assembly entry points fail with an error.
+ */
+#ifndef BEMPP_AMD_H
+#define BEMPP_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ndelement ReferenceCellType numbering (python/bempp/function_space.py:211-222). */
+enum {
+  BEMPP_CELL_POINT = 0,
+  BEMPP_CELL_INTERVAL = 1,
+  BEMPP_CELL_TRIANGLE = 2,
+  BEMPP_CELL_QUADRILATERAL = 3,
+  BEMPP_CELL_TETRAHEDRON = 4,
+  BEMPP_CELL_HEXAHEDRON = 5,
+  BEMPP_CELL_PRISM = 6,
+  BEMPP_CELL_PYRAMID = 7
+};
+
+/* DType (python/bempp/assembly/boundary.py:13-18; F32=0, F64=1 at function_space.py:13-16). */
+enum { BEMPP_DTYPE_F32 = 0, BEMPP_DTYPE_F64 = 1, BEMPP_DTYPE_C32 = 2, BEMPP_DTYPE_C64 = 3 };
+
+/* OperatorType, #[repr(u8)] (src/bindings/boundary_assemblers.rs:15-24). */
+enum {
+  BEMPP_OP_SINGLE_LAYER = 0,
+  BEMPP_OP_DOUBLE_LAYER = 1,
+  BEMPP_OP_ADJOINT_DOUBLE_LAYER = 2,
+  BEMPP_OP_HYPERSINGULAR = 3,
+  BEMPP_OP_ELECTRIC_FIELD = 4,
+  BEMPP_OP_MAGNETIC_FIELD = 5
+};
+
+/* Continuity (ndelement; python/bempp/function_space.py:335-338). */
+enum { BEMPP_CONTINUITY_STANDARD = 0, BEMPP_CONTINUITY_DISCONTINUOUS = 1 };
+
+/* Precision modes (no counterpart in the reference). FP64: every pair in FP64.
+ * MIXED: regular far-field pairs evaluated in FP32 on pair-recentred
+ * coordinates with FP64 coordinate differences and phase reduction; Duffy and
+ * near pairs stay FP64; accumulation FP64. */
+enum { BEMPP_PRECISION_FP64 = 0, BEMPP_PRECISION_MIXED = 1 };
+
+typedef struct BoundaryAssemblerOptionsT BoundaryAssemblerOptionsT;
+typedef struct BoundaryAssemblerT BoundaryAssemblerT;
+typedef struct GridT GridT;
+typedef struct SpaceT SpaceT;
+typedef struct MatrixT MatrixT;
+
+/* Last error message of the calling thread (no counterpart: the reference panics). */
+const char* bempp_last_error(void);
+
+/* Library version string (no counterpart in the reference). */
+const char* bempp_version(void);
+
+/* Number of visible HIP devices; 0 when no GPU is usable (no counterpart). */
+int bempp_device_count(void);
+
+/* ---- options: src/bindings/boundary_assemblers.rs:34-125 ---- */
+/* replaces boundary_assembler_options_new (:34-40) */
+BoundaryAssemblerOptionsT* boundary_assembler_options_new(void);
+/* replaces boundary_assembler_options_t_free (:42-50) */
+void boundary_assembler_options_t_free(BoundaryAssemblerOptionsT* options);
+/* replaces boundary_assembler_options_set_regular_quadrature_degree (:52-60); like
+ * the reference it only changes cell types that already have an entry (:61-65). */
+void boundary_assembler_options_set_regular_quadrature_degree(BoundaryAssemblerOptionsT* options, uint8_t cell_type,
+                                                               size_t npoints);
+/* replaces ..._get_regular_quadrature_degree (:61-68); returns 0 when absent (reference panics). */
+size_t boundary_assembler_options_get_regular_quadrature_degree(const BoundaryAssemblerOptionsT* options,
+                                                                uint8_t cell_type);
+/* replaces ..._has_regular_quadrature_degree (:69-76) */
+int boundary_assembler_options_has_regular_quadrature_degree(const BoundaryAssemblerOptionsT* options,
+                                                             uint8_t cell_type);
+/* replaces ..._set_singular_quadrature_degree (:78-87) */
+void boundary_assembler_options_set_singular_quadrature_degree(BoundaryAssemblerOptionsT* options,
+                                                                uint8_t cell_type0, uint8_t cell_type1,
+                                                                size_t npoints);
+/* replaces ..._get_singular_quadrature_degree (:88-98); returns 0 when absent. */
+size_t boundary_assembler_options_get_singular_quadrature_degree(const BoundaryAssemblerOptionsT* options,
+                                                                 uint8_t cell_type0, uint8_t cell_type1);
+/* replaces ..._has_singular_quadrature_degree (:99-109) */
+int boundary_assembler_options_has_singular_quadrature_degree(const BoundaryAssemblerOptionsT* options,
+                                                              uint8_t cell_type0, uint8_t cell_type1);
+/* replaces ..._set_batch_size (:111-118) */
+void boundary_assembler_options_set_batch_size(BoundaryAssemblerOptionsT* options, size_t batch_size);
+/* replaces ..._get_batch_size (:119-125) */
+size_t boundary_assembler_options_get_batch_size(const BoundaryAssemblerOptionsT* options);
+/* Precision mode (no counterpart in the reference); returns non-zero on a bad mode. */
+int boundary_assembler_options_set_precision(BoundaryAssemblerOptionsT* options, uint8_t precision);
+uint8_t boundary_assembler_options_get_precision(const BoundaryAssemblerOptionsT* options);
+
+/* ---- grids and spaces (the reference binds ndgrid/ndelement objects,
+ *      src/bindings/ndgrid.rs:16-56, src/bindings/space.rs:20-42; here a grid is
+ *      built from plain arrays of flat triangles) ---- */
+/* A grid of flat triangles: points [npoints][3] f64, cells [ncells][3] vertex ids. */
+GridT* grid_new(const double* points, size_t npoints, const int64_t* cells, size_t ncells);
+/* Octahedron sphere of src/shapes.rs:19-114 (seed 0) or an icosahedron refined
+ * by the same scheme (seed 1). */
+GridT* grid_new_sphere(int seed, uint32_t refinement_level);
+void grid_t_free(GridT* grid);
+size_t grid_npoints(const GridT* grid);
+size_t grid_ncells(const GridT* grid);
+void grid_points_copy(const GridT* grid, double* points);
+void grid_cells_copy(const GridT* grid, int64_t* cells);
+/* Replace the vertex coordinates (same topology); drops cached device geometry. */
+int grid_set_points(GridT* grid, const double* points);
+
+/* replaces function_space(grid, family) (src/bindings/space.rs:20-42) with the
+ * Lagrange family given by degree (0 or 1) and continuity. */
+SpaceT* function_space_new(const GridT* grid, size_t degree, uint8_t continuity);
+void space_t_free(SpaceT* space);
+/* Space accessors (python/bempp/function_space.py:256-311) */
+size_t space_global_size(const SpaceT* space);
+size_t space_local_size(const SpaceT* space);
+int space_is_serial(const SpaceT* space);
+size_t space_cell_dofs_size(const SpaceT* space, size_t cell);
+int space_has_cell_dofs(const SpaceT* space, size_t cell);
+void space_cell_dofs(const SpaceT* space, size_t cell, uintptr_t* dofs);
+size_t space_global_dof_index(const SpaceT* space, size_t local_dof_index);
+/* Greedy cell colouring of src/function.rs:281-351: colour of each cell, and count. */
+size_t space_cell_colouring(const SpaceT* space, int64_t* cell_colour);
+
+/* ---- assemblers: src/bindings/boundary_assemblers.rs:127-347 ---- */
+/* replaces laplace_boundary_assembler_new (:127-166); dtype F32 or F64. */
+BoundaryAssemblerT* laplace_boundary_assembler_new(const BoundaryAssemblerOptionsT* options, uint8_t operator_type,
+                                                   uint8_t dtype);
+/* replaces helmholtz_boundary_assembler_new (:168-210); k points to a float
+ * (C32) or double (C64). */
+BoundaryAssemblerT* helmholtz_boundary_assembler_new(const void* k, const BoundaryAssemblerOptionsT* options,
+                                                     uint8_t operator_type, uint8_t dtype);
+/* replaces boundary_assembler_t_free (generated by #[cfuncs], :26-27) */
+void boundary_assembler_t_free(BoundaryAssemblerT* assembler);
+/* replaces boundary_assembler_dtype (:212-233) */
+uint8_t boundary_assembler_dtype(const BoundaryAssemblerT* assembler);
+
+/* replaces boundary_assembler_assemble_into_memory (:309-347): host buffer of
+ * n_test*n_trial elements of the assembler's dtype, Fortran order, accumulated into. */
+int boundary_assembler_assemble_into_memory(const BoundaryAssemblerT* assembler, const SpaceT* trial_space,
+                                            const SpaceT* test_space, void* data);
+
+/* Device-resident variant (no counterpart in the reference): `data` is device
+ * memory on the current HIP device. accumulate != 0 adds into it, otherwise it
+ * is overwritten. `stream` is a hipStream_t (NULL: the assembler's own stream).
+ * The call returns once the work is enqueued; synchronize the stream to wait. */
+int boundary_assembler_assemble_into_device_memory(const BoundaryAssemblerT* assembler, const SpaceT* trial_space,
+                                                   const SpaceT* test_space, void* data, int accumulate,
+                                                   void* stream);
+
+/* replaces boundary_assembler_assemble (:272-307): dense matrix handle. */
+MatrixT* boundary_assembler_assemble(const BoundaryAssemblerT* assembler, const SpaceT* trial_space,
+                                     const SpaceT* test_space);
+/* replaces boundary_assembler_assemble_singular (:235-270): CSR matrix of the
+ * adjacent-pair (Duffy) contributions with duplicates summed; empty when every
+ * |entry| <= 1e-10 (src/boundary_assemblers.rs:122-155). */
+MatrixT* boundary_assembler_assemble_singular(const BoundaryAssemblerT* assembler, const SpaceT* trial_space,
+                                              const SpaceT* test_space);
+
+/* Shape gradient (no counterpart in the reference): grad[v][d] = d(v^T A(X) u)/dX[v][d]
+ * for the operator of `assembler` on one grid (test and trial space on the same
+ * grid), computed matrix-free on the device with recomputation. u has n_trial,
+ * v n_test entries of the assembler's dtype; grad is real f64 [npoints][3] for
+ * Laplace and complex128 [npoints][3] for Helmholtz (d/dX of the complex bilinear form). */
+int boundary_assembler_shape_gradient(const BoundaryAssemblerT* assembler, const SpaceT* trial_space,
+                                      const SpaceT* test_space, const void* u, const void* v, void* grad);
+
+/* Kernel timing (no counterpart in the reference): with profiling enabled, each
+ * assembly records HIP events around its regular-pair and adjacent-pair kernels
+ * on the launch stream; kernel_times waits for them and writes the elapsed
+ * milliseconds (ms[0] regular, ms[1] adjacent; -1 when a kernel did not run). */
+int boundary_assembler_set_profiling(BoundaryAssemblerT* assembler, int enable);
+int boundary_assembler_kernel_times(const BoundaryAssemblerT* assembler, float* ms);
+
+/* ---- matrices: src/bindings/matrices.rs:6-138 ---- */
+size_t matrix_shape0(const MatrixT* matrix);
+size_t matrix_shape1(const MatrixT* matrix);
+/* 0 dense, 1 CSR (no counterpart: the reference dispatches on the boxed type) */
+int matrix_is_csr(const MatrixT* matrix);
+uint8_t matrix_dtype(const MatrixT* matrix);
+const void* matrix_dense_data(const MatrixT* matrix);
+void matrix_dense_data_copy(const MatrixT* matrix, void* data);
+size_t matrix_csr_data_size(const MatrixT* matrix);
+size_t matrix_csr_indices_size(const MatrixT* matrix);
+size_t matrix_csr_indptr_size(const MatrixT* matrix);
+void matrix_csr_data_copy(const MatrixT* matrix, void* data);
+void matrix_csr_indices_copy(const MatrixT* matrix, uintptr_t* indices);
+void matrix_csr_indptr_copy(const MatrixT* matrix, uintptr_t* indptr);
+void matrix_t_free(MatrixT* matrix);
+
+/* ---- quadrature tables (no counterpart in the reference's bindings; they expose
+ *      what ndelement::quadrature::simplex_rule and
+ *      bempp_quadrature::duffy::triangle_duffy provide to src/boundary_assemblers.rs:419-464,524-562) ---- */
+/* Symmetric rule on the reference triangle: xi [npoints][2], w [npoints]; returns
+ * npoints, or -1 for an unsupported count (supported: 1, 3, 6, 7). */
+int bempp_triangle_rule(int npoints, double* xi, double* w);
+/* Canonical Sauter-Schwab rule, nshared = 1 (vertex), 2 (edge), 3 (coincident),
+ * n Gauss points per dimension: pts5 [npts][5] = test xi0, xi1, trial xi0, xi1,
+ * weight. Returns npts (pass pts5 = NULL to query it). */
+int bempp_canonical_pair_rule(int nshared, int n, double* pts5);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BEMPP_AMD_H */

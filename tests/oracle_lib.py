@@ -1,0 +1,137 @@
+"""ctypes access to the CPU oracle (oracle/_build/libbem_oracle.so) for tests and the benchmark's
+cpu_baseline leg. The oracle is the checker: nothing in the product imports this module."""
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent.parent
+LIB_PATH = ROOT / "oracle" / "_build" / "libbem_oracle.so"
+
+OPS = {"single_layer": 0, "double_layer": 1, "adjoint_double_layer": 2, "hypersingular": 3}
+KINDS = {"laplace": 0, "helmholtz": 1}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run __graft_entry__.build())")
+        L = ctypes.CDLL(str(LIB_PATH))
+        i64, i32, dbl, vp = ctypes.c_int64, ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+        L.oracle_last_error.restype = ctypes.c_char_p
+        L.oracle_assemble_dense.argtypes = [i32, i32, dbl, vp, i64, vp, i64, i32, i32, i32, i32, i32, i32, i32,
+                                            i32, i32, i64, i64, vp]
+        L.oracle_assemble_singular_coo.argtypes = [i32, i32, dbl, vp, i64, vp, i64, i32, i32, i32, i32, i32, i32,
+                                                   i32, vp, vp, vp, vp]
+        L.oracle_sphere.argtypes = [i32, i32, vp, vp, vp, vp]
+        L.oracle_space.argtypes = [vp, i64, vp, i64, i32, i32, vp, vp, vp, vp]
+        L.oracle_regular_rule.argtypes = [i32, vp, vp]
+        L.oracle_gauss_legendre01.argtypes = [i32, vp, vp]
+        L.oracle_duffy_rule.argtypes = [i32, vp, vp, i32, vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _check(rc):
+    if rc != 0:
+        raise ValueError(lib().oracle_last_error().decode())
+
+
+def sphere(level, seed="octahedron"):
+    """Unit sphere mesh: seed 'octahedron' (src/shapes.rs:19-114) or 'icosahedron'."""
+    s = 0 if seed == "octahedron" else 1
+    nv, nc = ctypes.c_int64(), ctypes.c_int64()
+    _check(lib().oracle_sphere(s, level, ctypes.byref(nv), ctypes.byref(nc), None, None))
+    pts = np.zeros((nv.value, 3))
+    cells = np.zeros((nc.value, 3), dtype=np.int64)
+    _check(lib().oracle_sphere(s, level, None, None, _p(pts), _p(cells)))
+    return pts, cells
+
+
+def space(pts, cells, degree, continuous):
+    pts = np.ascontiguousarray(pts, dtype=np.float64)
+    cells = np.ascontiguousarray(cells, dtype=np.int64)
+    nb = 1 if degree == 0 else 3
+    size, ncol = ctypes.c_int64(), ctypes.c_int64()
+    dofs = np.zeros((len(cells), nb), dtype=np.int64)
+    colour = np.zeros(len(cells), dtype=np.int64)
+    _check(lib().oracle_space(_p(pts), len(pts), _p(cells), len(cells), degree, int(continuous),
+                              ctypes.byref(size), _p(dofs), _p(colour), ctypes.byref(ncol)))
+    return size.value, dofs, colour, ncol.value
+
+
+def _space_size(pts, cells, degree, continuous):
+    return space(pts, cells, degree, continuous)[0]
+
+
+def assemble(kind, op, pts, cells, test=(0, False), trial=(0, False), k=0.0, nq=6, nsing=4, batch=128,
+             nthreads=0, fast=False, rows=None, out=None):
+    """Dense assembly (accumulated into `out` if given). test/trial = (degree, continuous)."""
+    pts = np.ascontiguousarray(pts, dtype=np.float64)
+    cells = np.ascontiguousarray(cells, dtype=np.int64)
+    nt = _space_size(pts, cells, *test)
+    ns = _space_size(pts, cells, *trial)
+    rb, re = (0, len(cells)) if rows is None else rows
+    if rows is not None:
+        nt = re - rb
+    dtype = np.float64 if kind == "laplace" else np.complex128
+    if out is None:
+        out = np.zeros((nt, ns), dtype=dtype, order="F")
+    assert out.flags.f_contiguous and out.dtype == dtype and out.shape == (nt, ns)
+    _check(lib().oracle_assemble_dense(KINDS[kind], OPS[op], float(k), _p(pts), len(pts), _p(cells), len(cells),
+                                       test[0], int(test[1]), trial[0], int(trial[1]), nq, nsing, batch, nthreads,
+                                       int(fast), rb, re, _p(out)))
+    return out
+
+
+def assemble_singular_coo(kind, op, pts, cells, test=(0, False), trial=(0, False), k=0.0, nsing=4, batch=128,
+                          nthreads=0):
+    pts = np.ascontiguousarray(pts, dtype=np.float64)
+    cells = np.ascontiguousarray(cells, dtype=np.int64)
+    cnt = ctypes.c_int64()
+    args = [KINDS[kind], OPS[op], float(k), _p(pts), len(pts), _p(cells), len(cells), test[0], int(test[1]),
+            trial[0], int(trial[1]), nsing, batch, nthreads]
+    _check(lib().oracle_assemble_singular_coo(*args, ctypes.byref(cnt), None, None, None))
+    n = cnt.value
+    rows = np.zeros(n, dtype=np.int64)
+    cols = np.zeros(n, dtype=np.int64)
+    vals = np.zeros(n, dtype=np.float64 if kind == "laplace" else np.complex128)
+    _check(lib().oracle_assemble_singular_coo(*args, ctypes.byref(cnt), _p(rows), _p(cols), _p(vals)))
+    return rows, cols, vals
+
+
+def regular_rule(npts):
+    p = np.zeros(2 * max(npts, 1))
+    w = np.zeros(max(npts, 1))
+    q = lib().oracle_regular_rule(npts, _p(p), _p(w))
+    if q < 0:
+        raise ValueError(lib().oracle_last_error().decode())
+    return p.reshape(-1, 2)[:q], w[:q]
+
+
+def gauss_legendre01(n):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    lib().oracle_gauss_legendre01(n, _p(x), _p(w))
+    return x, w
+
+
+def duffy_rule(pairs, n):
+    tl = np.array([a for a, _ in pairs], dtype=np.int32)
+    sl = np.array([b for _, b in pairs], dtype=np.int32)
+    npts = lib().oracle_duffy_rule(len(pairs), _p(tl), _p(sl), n, None, None, None)
+    if npts < 0:
+        raise ValueError(lib().oracle_last_error().decode())
+    tp = np.zeros((npts, 2))
+    sp = np.zeros((npts, 2))
+    w = np.zeros(npts)
+    lib().oracle_duffy_rule(len(pairs), _p(tl), _p(sl), n, _p(tp), _p(sp), _p(w))
+    return tp, sp, w

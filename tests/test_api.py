@@ -1,0 +1,156 @@
+"""Host-side behaviour of the C ABI and the python mirror (no GPU needed)."""
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import bempp_amd
+from bempp_amd import _lib as L
+from bempp_amd.assembly.boundary import (BoundaryAssemblerOptions, OperatorType, Precision,
+                                         create_helmholtz_assembler, create_laplace_assembler)
+from bempp_amd.element import Continuity, Family, ReferenceCellType, create_family
+from bempp_amd.function_space import function_space
+from bempp_amd.shapes import icosphere, regular_sphere
+
+import oracle_lib as O
+
+HEADER = Path(__file__).resolve().parent.parent / "include" / "bempp_amd.h"
+
+
+def test_library_exports_every_declared_symbol():
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\**\s+\**([a-z_][a-z0-9_]*)\(", HEADER.read_text(),
+                       flags=re.M)
+    assert len(names) > 50
+    lib = ctypes.CDLL(L.lib_path())
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+@pytest.mark.parametrize("otype", [OperatorType.SingleLayer, OperatorType.DoubleLayer,
+                                   OperatorType.AdjointDoubleLayer, OperatorType.Hypersingular])
+def test_create_assembler(otype):
+    """python/test/test_boundary_assembler.py:10-42."""
+    o = BoundaryAssemblerOptions()
+    assert o.regular_quadrature_degree(ReferenceCellType.Triangle) != 3
+    o.set_regular_quadrature_degree(ReferenceCellType.Triangle, 3)
+    assert o.regular_quadrature_degree(ReferenceCellType.Triangle) == 3
+    with pytest.raises(ValueError):
+        o.set_regular_quadrature_degree(ReferenceCellType.Interval, 3)
+    with pytest.raises(ValueError):
+        o.regular_quadrature_degree(ReferenceCellType.Interval)
+    assert o.singular_quadrature_degree(ReferenceCellType.Triangle, ReferenceCellType.Triangle) != 3
+    o.set_singular_quadrature_degree(ReferenceCellType.Triangle, ReferenceCellType.Triangle, 3)
+    assert o.singular_quadrature_degree(ReferenceCellType.Triangle, ReferenceCellType.Triangle) == 3
+    with pytest.raises(ValueError):
+        o.set_singular_quadrature_degree(ReferenceCellType.Interval, ReferenceCellType.Interval, 3)
+    assert o.batch_size() != 4
+    o.set_batch_size(4)
+    assert o.batch_size() == 4
+    a = create_laplace_assembler(otype, o)
+    assert a.dtype == np.float64
+    with pytest.raises(ValueError):
+        o.set_batch_size(8)  # read-only once used (boundary.py:117-118)
+
+
+def test_defaults_and_precision():
+    o = BoundaryAssemblerOptions()
+    assert o.regular_quadrature_degree(ReferenceCellType.Triangle) == 37  # src/boundary_assemblers.rs:47
+    assert o.regular_quadrature_degree(ReferenceCellType.Quadrilateral) == 37
+    assert o.singular_quadrature_degree(ReferenceCellType.Triangle, ReferenceCellType.Triangle) == 4
+    assert o.singular_quadrature_degree(ReferenceCellType.Quadrilateral, ReferenceCellType.Triangle) == 4
+    assert o.batch_size() == 128
+    assert o.precision() == Precision.FP64
+    o.set_precision(Precision.Mixed)
+    assert o.precision() == Precision.Mixed
+
+
+def test_invalid_assemblers():
+    """The reference panics on these (src/bindings/boundary_assemblers.rs:153-164,197-208)."""
+    with pytest.raises(ValueError):
+        create_laplace_assembler(OperatorType.ElectricField)
+    with pytest.raises(ValueError):
+        create_laplace_assembler(OperatorType.SingleLayer, dtype=np.complex128)
+    with pytest.raises(ValueError):
+        create_helmholtz_assembler(1.0, OperatorType.SingleLayer, dtype=np.float64)
+    assert create_helmholtz_assembler(1.0, OperatorType.DoubleLayer, dtype=np.complex64).dtype == np.complex64
+
+
+@pytest.mark.parametrize("level", range(4))
+def test_spaces(level):
+    """python/test/test_function_space.py:8-41 and tests/test_space.rs:137-164."""
+    g = regular_sphere(level)
+    dp0 = function_space(g, create_family(Family.Lagrange, 0, Continuity.Discontinuous))
+    p1 = function_space(g, create_family(Family.Lagrange, 1))
+    assert dp0.global_size == dp0.local_size == g.entity_count(ReferenceCellType.Triangle)
+    assert p1.global_size == g.entity_count(ReferenceCellType.Point)
+    assert dp0.is_serial and p1.is_serial
+    assert g.topology_dim == 2 and g.geometry_dim == 3
+    assert len(dp0.cell_colouring()) == 1
+    for cells in p1.cell_colouring():
+        verts = g.cells[cells].ravel()
+        assert len(verts) == len(np.unique(verts))
+    assert dp0.cell_dofs(0) == [0] and dp0.cell_dofs(10 ** 9) is None
+
+
+def test_spaces_match_oracle():
+    """DOF numbering and colouring agree with the oracle's restatement of src/function.rs."""
+    g = icosphere(2)
+    p, c = g.points, g.cells
+    po, co = O.sphere(2, "icosahedron")
+    assert np.abs(p - po).max() < 1e-15 and np.array_equal(c, co)  # FMA contraction may differ by 1 ulp
+    for deg, cont in [(0, False), (1, True), (1, False)]:
+        s = function_space(g, create_family(Family.Lagrange, deg,
+                                            Continuity.Standard if cont else Continuity.Discontinuous))
+        size, dofs, colour, ncol = O.space(p, c, deg, cont)
+        assert s.global_size == size
+        assert np.array_equal(np.array([s.cell_dofs(i) for i in range(len(c))]), dofs)
+        cols = s.cell_colouring()
+        assert len(cols) == ncol
+        for k, cells in enumerate(cols):
+            assert np.all(colour[cells] == k)
+
+
+def test_octahedron_matches_reference_shape():
+    p, c = O.sphere(3)
+    g = regular_sphere(3)
+    assert np.abs(g.points - p).max() < 1e-15 and np.array_equal(g.cells, c)
+
+
+def test_rules_match_oracle():
+    """The engine's rule tables equal the oracle's (triangle rules and canonical Duffy rules)."""
+    lib = L.lib()
+    for q in (1, 3, 6, 7):
+        xi = np.zeros(2 * q)
+        w = np.zeros(q)
+        assert lib.bempp_triangle_rule(q, L.ptr(xi), L.ptr(w)) == q
+        po, wo = O.regular_rule(q)
+        assert np.abs(xi.reshape(-1, 2) - po).max() < 5e-16 and np.abs(w - wo).max() < 5e-17
+    assert lib.bempp_triangle_rule(37, None, None) == -1
+    canonical = {1: [(0, 0)], 2: [(0, 0), (1, 1)], 3: [(0, 0), (1, 1), (2, 2)]}
+    for m, pairs in canonical.items():
+        n = lib.bempp_canonical_pair_rule(m, 4, None)
+        r = np.zeros((n, 5))
+        lib.bempp_canonical_pair_rule(m, 4, L.ptr(r))
+        tp, sp, w = O.duffy_rule(pairs, 4)
+        assert np.abs(r[:, 0:2] - tp).max() < 5e-16 and np.abs(r[:, 2:4] - sp).max() < 5e-16
+        assert np.abs(r[:, 4] / w - 1).max() < 1e-14  # same formula, different product order
+
+
+def test_bad_grids():
+    with pytest.raises(ValueError):
+        bempp_amd.grid.Grid(np.zeros((3, 3)), np.array([[0, 1, 5]]))
+    with pytest.raises(ValueError):
+        bempp_amd.grid.Grid(np.zeros((3, 3)), np.array([[0, 1, 1]]))
+
+
+def test_no_gpu_fails_loudly():
+    if bempp_amd.device_count() > 0:
+        pytest.skip("a GPU is present")
+    g = regular_sphere(0)
+    s = function_space(g, create_family(Family.Lagrange, 0, Continuity.Discontinuous))
+    o = BoundaryAssemblerOptions()
+    o.set_regular_quadrature_degree(ReferenceCellType.Triangle, 6)
+    with pytest.raises(ValueError, match="no HIP device"):
+        create_laplace_assembler(OperatorType.SingleLayer, o).assemble(s, s)
