@@ -87,6 +87,13 @@ def lib():
     """Load the native library; raises if it has not been built (there is no fallback)."""
     global _lib
     if _lib is None:
+        # torch bundles its own HIP runtime (same soname as the system one). Loading
+        # torch first makes libbempp_amd bind to that runtime, so device memory and
+        # streams are shared with torch instead of two runtimes competing for the GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not _LIB_FILE.exists():
             raise ImportError(f"{_LIB_FILE} is missing: build it with __graft_entry__.build() "
                               "(make -C bempp-rs_amd/csrc); bempp_amd has no non-native fallback")

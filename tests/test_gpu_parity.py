@@ -47,6 +47,11 @@ def _check(kind, op, grid, test, trial, k=0.0, nq=6, ns=4, tol=TOL):
     print(f"[parity] {kind} {op} test={test} trial={trial} ncells={len(c)} k={k} nq={nq}", flush=True)
     A = a.assemble(ss, ts)
     R = O.assemble(kind, op, p, c, test=SPACES[test], trial=SPACES[trial], k=k, nq=nq, nsing=ns)
+    if np.linalg.norm(R) == 0.0:
+        # Laplace hypersingular with a P0 factor: the curl of a constant vanishes, A == 0
+        print("[parity]   operator is identically zero", flush=True)
+        assert np.abs(A).max() == 0.0
+        return A, R
     err = _rel(A, R)
     print(f"[parity]   rel err {err:.3e}", flush=True)
     assert err <= tol, err
@@ -107,7 +112,7 @@ def test_accumulates_into_memory():
     A = a.assemble(s, s)
     B = np.full_like(A, 1.0 + 2.0j, order="F")
     a.assemble_into_memory(s, s, B)
-    assert np.abs(B - (A + 1.0 + 2.0j)).max() < 1e-15
+    assert np.abs(B - (A + 1.0 + 2.0j)).max() < 1e-13  # atomics: summation order varies
 
 
 def test_singular_csr_matches_oracle():
