@@ -8,6 +8,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <tuple>
 
 #include "core.hpp"
 #include "kernels.hpp"
@@ -31,6 +32,7 @@ struct DeviceGridCache {
   int4* pairs = nullptr;          // adjacent pairs
   int npairs = -1;
   std::vector<AdjacentPair> host_pairs;
+  double max_radius = 0.0;        // largest centroid-to-vertex distance of any cell
 };
 
 struct DeviceSpaceCache {
@@ -109,6 +111,16 @@ void cell_record(const Grid& g, int64_t c, double* r) {
   }
   r[12] = jdet;
   r[22] = r[23] = 0.0;
+  double rad2 = 0.0;
+  for (int d = 0; d < 3; ++d) r[dev::kGeoCentroid + d] = (v[0][d] + v[1][d] + v[2][d]) / 3.0;
+  for (int m = 0; m < 3; ++m) {
+    double s = 0.0;
+    for (int d = 0; d < 3; ++d) s += (v[m][d] - r[dev::kGeoCentroid + d]) * (v[m][d] - r[dev::kGeoCentroid + d]);
+    rad2 = std::max(rad2, s);
+  }
+  // slightly enlarged so that rounding never lets a quadrature point escape the bound
+  r[dev::kGeoRadius] = std::sqrt(rad2) * (1.0 + 1e-12);
+  for (int m = dev::kGeoRadius + 1; m < dev::kGeoStride; ++m) r[m] = 0.0;
 }
 
 DeviceGridCache* grid_cache(Grid& g, int device) {
@@ -122,6 +134,7 @@ DeviceGridCache* grid_cache(Grid& g, int device) {
   std::vector<int> vid(nc * 4);
   for (int64_t i = 0; i < nc; ++i) {
     cell_record(g, i, &geo[i * dev::kGeoStride]);
+    c->max_radius = std::max(c->max_radius, geo[i * dev::kGeoStride + dev::kGeoRadius]);
     for (int m = 0; m < 3; ++m) vid[4 * i + m] = (int)g.cells[3 * i + m];
     vid[4 * i + 3] = -1;
   }
@@ -170,6 +183,9 @@ const int4* grid_pairs(Grid& g, DeviceGridCache* c, int* npairs) {
   std::lock_guard<std::mutex> lk(g.cache_mutex);
   if (c->npairs < 0) {
     adjacent_pairs(g, c->host_pairs);
+    // group by adjacency type: consecutive wavefronts then read the same rule table
+    std::stable_sort(c->host_pairs.begin(), c->host_pairs.end(),
+                     [](const AdjacentPair& x, const AdjacentPair& y) { return x.nshared < y.nshared; });
     std::vector<int4> p(c->host_pairs.size());
     for (size_t i = 0; i < p.size(); ++i) {
       const AdjacentPair& a = c->host_pairs[i];
@@ -267,54 +283,93 @@ int singular_points(const Options& o) {
   return (int)it->second;
 }
 
-// ---- kernel dispatch tables ----
-template <int OP, bool HELM, bool P0P0>
-void launch_regular_q(int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& A) {
-  switch (Q) {
-    case 1: dev::regular_kernel<1, OP, HELM, P0P0><<<grid, 256, 0, s>>>(A); break;
-    case 3: dev::regular_kernel<3, OP, HELM, P0P0><<<grid, 256, 0, s>>>(A); break;
-    case 6: dev::regular_kernel<6, OP, HELM, P0P0><<<grid, 256, 0, s>>>(A); break;
-    case 7: dev::regular_kernel<7, OP, HELM, P0P0><<<grid, 256, 0, s>>>(A); break;
-    default: throw Error("unsupported regular quadrature point count");
+// Tile coordinates (tile row = test block, tile column = trial block) for the
+// P0 x P0 kernel: all tiles, or the upper triangle a <= b when symmetric.
+// Cached per device and shape for the life of the process.
+const int2* tile_list(int device, int nta, int ntb, bool symmetric, int* count) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int, bool>, std::pair<int2*, int>> cache;
+  std::lock_guard<std::mutex> lk(mu);
+  auto key = std::make_tuple(device, nta, ntb, symmetric);
+  auto it = cache.find(key);
+  if (it == cache.end()) {
+    std::vector<int2> t;
+    for (int b = 0; b < ntb; ++b)
+      for (int a = 0; a < (symmetric ? std::min(b + 1, nta) : nta); ++a) t.push_back(make_int2(a, b));
+    it = cache.emplace(key, std::make_pair(upload(t), (int)t.size())).first;
   }
+  *count = it->second.second;
+  return it->second.first;
 }
 
-template <bool HELM, bool P0P0>
-void launch_regular_op(int op, int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& A) {
+// ---- kernel dispatch: tiled P0 x P0 or general, by operator, kernel and phase mode ----
+template <int OP, bool HELM, bool CENTRED>
+void launch_q(bool tiled, int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& A, const int2* tiles, int sym) {
+#define BEMPP_LAUNCH(QQ)                                                                 \
+  if (tiled) dev::regular_p0_tiled<QQ, OP, HELM, CENTRED><<<grid, 256, 0, s>>>(A, tiles, sym); \
+  else dev::regular_kernel<QQ, OP, HELM, CENTRED><<<grid, 256, 0, s>>>(A);
+  switch (Q) {
+    case 1: BEMPP_LAUNCH(1); break;
+    case 3: BEMPP_LAUNCH(3); break;
+    case 6: BEMPP_LAUNCH(6); break;
+    case 7: BEMPP_LAUNCH(7); break;
+    default: throw Error("unsupported regular quadrature point count");
+  }
+#undef BEMPP_LAUNCH
+}
+
+template <bool HELM, bool CENTRED>
+void launch_op(int op, bool tiled, int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& A, const int2* tiles,
+               int sym) {
   switch (op) {
-    case kSL: launch_regular_q<kSL, HELM, P0P0>(Q, grid, s, A); break;
-    case kDL: launch_regular_q<kDL, HELM, P0P0>(Q, grid, s, A); break;
-    case kADL: launch_regular_q<kADL, HELM, P0P0>(Q, grid, s, A); break;
-    case kHS: launch_regular_q<kHS, HELM, P0P0>(Q, grid, s, A); break;
+    case kSL: launch_q<kSL, HELM, CENTRED>(tiled, Q, grid, s, A, tiles, sym); break;
+    case kDL: launch_q<kDL, HELM, CENTRED>(tiled, Q, grid, s, A, tiles, sym); break;
+    case kADL: launch_q<kADL, HELM, CENTRED>(tiled, Q, grid, s, A, tiles, sym); break;
+    case kHS: launch_q<kHS, HELM, CENTRED>(tiled, Q, grid, s, A, tiles, sym); break;
     default: throw Error("operator not supported by the dense assembler");
   }
 }
 
-void launch_regular(int op, bool helm, bool p0p0, int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& A) {
-  if (helm) {
-    if (p0p0) launch_regular_op<true, true>(op, Q, grid, s, A);
-    else launch_regular_op<true, false>(op, Q, grid, s, A);
-  } else {
-    if (p0p0) launch_regular_op<false, true>(op, Q, grid, s, A);
-    else launch_regular_op<false, false>(op, Q, grid, s, A);
-  }
+// Regular pairs: `tiled` selects the P0 x P0 tile kernel (grid = tile count),
+// otherwise the general kernel (grid = test blocks x trial chunks).
+void launch_regular(int op, bool helm, bool centred, bool tiled, int Q, dim3 grid, hipStream_t s,
+                    const dev::RegularArgs& A, const int2* tiles = nullptr, int sym = 0) {
+  if (!helm) launch_op<false, false>(op, tiled, Q, grid, s, A, tiles, sym);
+  else if (centred) launch_op<true, true>(op, tiled, Q, grid, s, A, tiles, sym);
+  else launch_op<true, false>(op, tiled, Q, grid, s, A, tiles, sym);
   HIP_CHECK(hipGetLastError());
 }
 
-template <bool HELM>
-void launch_singular_op(int op, dim3 grid, hipStream_t s, const dev::SingularArgs& A) {
+template <bool HELM, bool P0P0, bool SMALL>
+void launch_singular_t(int op, dim3 grid, hipStream_t s, const dev::SingularArgs& A) {
   switch (op) {
-    case kSL: dev::singular_kernel<kSL, HELM><<<grid, 256, 0, s>>>(A); break;
-    case kDL: dev::singular_kernel<kDL, HELM><<<grid, 256, 0, s>>>(A); break;
-    case kADL: dev::singular_kernel<kADL, HELM><<<grid, 256, 0, s>>>(A); break;
-    case kHS: dev::singular_kernel<kHS, HELM><<<grid, 256, 0, s>>>(A); break;
+    case kSL: dev::singular_kernel<kSL, HELM, P0P0, SMALL><<<grid, 256, 0, s>>>(A); break;
+    case kDL: dev::singular_kernel<kDL, HELM, P0P0, SMALL><<<grid, 256, 0, s>>>(A); break;
+    case kADL: dev::singular_kernel<kADL, HELM, P0P0, SMALL><<<grid, 256, 0, s>>>(A); break;
+    case kHS: dev::singular_kernel<kHS, HELM, P0P0, SMALL><<<grid, 256, 0, s>>>(A); break;
     default: throw Error("operator not supported by the dense assembler");
+  }
+}
+
+void launch_singular(int op, bool helm, bool p0p0, bool small, dim3 grid, hipStream_t s,
+                     const dev::SingularArgs& A) {
+  if (!helm) {
+    if (p0p0) launch_singular_t<false, true, false>(op, grid, s, A);
+    else launch_singular_t<false, false, false>(op, grid, s, A);
+  } else if (small) {
+    if (p0p0) launch_singular_t<true, true, true>(op, grid, s, A);
+    else launch_singular_t<true, false, true>(op, grid, s, A);
+  } else {
+    if (p0p0) launch_singular_t<true, true, false>(op, grid, s, A);
+    else launch_singular_t<true, false, false>(op, grid, s, A);
   }
   HIP_CHECK(hipGetLastError());
 }
 
 struct Prepared {
   int device;
+  bool centred;         // Helmholtz phases taken relative to the cell-pair centroid distance
+  bool small_adjacent;  // |k| |x - y| <= 0.5 on every adjacent cell pair
   hipStream_t stream;
   int Q;
   bool same_grid;
@@ -347,9 +402,28 @@ Prepared prepare(const Assembler& a, const FunctionSpace& trial_c, const Functio
   const int nsing = singular_points(a.options);
   if (nsing < 1 || nsing > 64) throw Error("singular quadrature degree must be in [1, 64]");
   P.same_grid = test.grid == trial.grid;
+  // the device sine/cosine reduces k r exactly only while k r <= 1e8
+  if (a.kind == kHelmholtz) {
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (const Grid* g : {test.grid, trial.grid})
+      for (int64_t v = 0; v < g->nv(); ++v)
+        for (int d = 0; d < 3; ++d) {
+          lo[d] = std::min(lo[d], g->points[3 * v + d]);
+          hi[d] = std::max(hi[d], g->points[3 * v + d]);
+        }
+    double diag = 0.0;
+    for (int d = 0; d < 3; ++d) diag += (hi[d] - lo[d]) * (hi[d] - lo[d]);
+    if (!(std::fabs(a.k) * std::sqrt(diag) <= 1e8))
+      throw Error("|k| times the mesh diameter exceeds 1e8 (or is not finite)");
+  }
 
   DeviceGridCache* tc = grid_cache(tgrid, P.device);
   DeviceGridCache* sc = P.same_grid ? tc : grid_cache(sgrid, P.device);
+  // |k (r - R)| <= |k| (rad_test + rad_trial) for every quadrature-point pair of a
+  // regular cell pair; the small-argument sine/cosine needs this bound <= 0.5.
+  P.centred = a.kind == kHelmholtz && std::fabs(a.k) * (tc->max_radius + sc->max_radius) <= 0.5;
+  // points of two cells sharing a vertex are at most 2 (rad_i + rad_j) apart
+  P.small_adjacent = a.kind == kHelmholtz && P.same_grid && std::fabs(a.k) * 4.0 * tc->max_radius <= 0.5;
   dev::RegularArgs& R = P.R;
   R.test_pts = grid_points_for_rule(tgrid, tc, P.Q, xi, w);
   R.trial_pts = grid_points_for_rule(sgrid, sc, P.Q, xi, w);
@@ -440,7 +514,18 @@ void assemble_device(const Assembler& a, const FunctionSpace& trial, const Funct
   }
   if (R.n_test > 0 && R.n_trial > 0) {
     if (a.profiling) HIP_CHECK(hipEventRecord((hipEvent_t)am.events[0], P.stream));
-    launch_regular(a.op, a.kind == kHelmholtz, p0p0, P.Q, grid, P.stream, R);
+    if (p0p0) {
+      // symmetric operators on one space: upper-triangle tiles, mirrored writes
+      // (a P0 space is determined by its grid, so same grid means same space)
+      const bool sym = (a.op == kSL || a.op == kHS) && P.same_grid;
+      const int nta = (R.n_test + dev::kTile - 1) / dev::kTile, ntb = (R.n_trial + dev::kTile - 1) / dev::kTile;
+      int ntiles = 0;
+      const int2* tiles = tile_list(P.device, nta, ntb, sym, &ntiles);
+      launch_regular(a.op, a.kind == kHelmholtz, P.centred, true, P.Q, dim3(ntiles), P.stream, R, tiles,
+                     sym ? 1 : 0);
+    } else {
+      launch_regular(a.op, a.kind == kHelmholtz, P.centred, false, P.Q, grid, P.stream, R);
+    }
     if (a.profiling) {
       HIP_CHECK(hipEventRecord((hipEvent_t)am.events[1], P.stream));
       am.events_recorded[0] = true;
@@ -452,8 +537,7 @@ void assemble_device(const Assembler& a, const FunctionSpace& trial, const Funct
     S.mode = p0p0 ? dev::kAccumulate : dev::kAtomic;
     dim3 sg((unsigned)((P.npairs + 3) / 4));
     if (a.profiling) HIP_CHECK(hipEventRecord((hipEvent_t)am.events[2], P.stream));
-    if (a.kind == kHelmholtz) launch_singular_op<true>(a.op, sg, P.stream, S);
-    else launch_singular_op<false>(a.op, sg, P.stream, S);
+    launch_singular(a.op, a.kind == kHelmholtz, p0p0, P.small_adjacent, sg, P.stream, S);
     if (a.profiling) {
       HIP_CHECK(hipEventRecord((hipEvent_t)am.events[3], P.stream));
       am.events_recorded[1] = true;
@@ -508,8 +592,7 @@ void assemble_singular_coo(const Assembler& a, const FunctionSpace& trial, const
     S.mode = dev::kCoo;
     S.out = d;
     dim3 sg((unsigned)((P.npairs + 3) / 4));
-    if (a.kind == kHelmholtz) launch_singular_op<true>(a.op, sg, P.stream, S);
-    else launch_singular_op<false>(a.op, sg, P.stream, S);
+    launch_singular(a.op, a.kind == kHelmholtz, test.nb == 1 && trial.nb == 1, P.small_adjacent, sg, P.stream, S);
     HIP_CHECK(hipMemcpyAsync(h.data(), d, h.size() * sizeof(double), hipMemcpyDeviceToHost, P.stream));
     HIP_CHECK(hipStreamSynchronize(P.stream));
   } catch (...) {

@@ -17,11 +17,17 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "devmath.hpp"
+
 namespace bempp_amd {
 namespace dev {
 
 constexpr double kInv4Pi = 0.079577471545947667884441881686257181;  // 1/(4 pi)
-constexpr int kGeoStride = 24;  // v0 v1 v2 (9) | n (3) | jdet (1) | curlJ_b (9) | pad (2)
+// per-cell record: v0 v1 v2 (9) | n (3) | jdet (1) | curlJ_b (9) | pad (2) |
+//                  centroid (3) at 24 | radius (max centroid-vertex distance) at 27 | pad (4)
+constexpr int kGeoStride = 32;
+constexpr int kGeoCentroid = 24;
+constexpr int kGeoRadius = 27;
 constexpr int kMaxQ = 7;
 
 enum OutType { kOutF64 = 0, kOutC128 = 1, kOutF32 = 2, kOutC64 = 3 };
@@ -72,17 +78,25 @@ struct cplx {
 };
 
 // Kernel scalar (without the 1/(4 pi) factor) for displacement d = x - y.
-// nx: test normal, ny: trial normal.
-template <int OP, bool HELM>
+// nx: test normal, ny: trial normal. CENTRED (Helmholtz): the phase is taken
+// relative to kR, i.e. the result lacks the factor e^{ikR} that the caller
+// applies once per cell pair; requires |k r - kR| <= 0.5.
+template <int OP, bool HELM, bool CENTRED = false>
 __device__ __forceinline__ cplx kernel_scalar(double dx, double dy, double dz, double k, const double* nx,
-                                              const double* ny) {
+                                              const double* ny, double kR = 0.0) {
   const double r2 = dx * dx + dy * dy + dz * dz;
-  const double rinv = 1.0 / sqrt(r2);
+  const double rinv = rsqrt_nr(r2);
   cplx g;
   double kr = 0.0, s = 0.0, c = 1.0;
   if constexpr (HELM) {
-    kr = k * (r2 * rinv);
-    sincos(kr, &s, &c);
+    const double r = r2 * rinv;
+    if constexpr (CENTRED) {
+      sincos_small(fma(k, r, -kR), &s, &c);
+      if constexpr (OP == 1 || OP == 2) kr = k * r;
+    } else {
+      kr = k * r;
+      sincos_cw(kr, &s, &c);
+    }
     g = {c * rinv, s * rinv};
   } else {
     g = {rinv, 0.0};
@@ -173,105 +187,235 @@ __device__ __forceinline__ cplx local_entry(const cplx M[3][3], int a, int b, cp
   return {v.re * kInv4Pi, v.im * kInv4Pi};
 }
 
+// Read-only view in the constant address space: lets the compiler fetch
+// wave-uniform data (the trial cell) with scalar loads into SGPRs.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T* constant_view(const T* p) {
+  return (const __attribute__((address_space(4))) T*)p;
+}
+
+// e^{ikR} for the centroid distance R of a cell pair (CENTRED Helmholtz pairs).
+__device__ __forceinline__ cplx pair_phase(const double* ct, const double* cs, double k, double* kR) {
+  const double dx = ct[0] - cs[0], dy = ct[1] - cs[1], dz = ct[2] - cs[2];
+  const double R2 = dx * dx + dy * dy + dz * dz;
+  const double R = R2 * rsqrt_nr(R2);
+  *kR = k * R;
+  cplx e;
+  sincos_cw(*kR, &e.im, &e.re);
+  return e;
+}
+
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) {
+  return {fma(a.re, b.re, -a.im * b.im), fma(a.re, b.im, a.im * b.re)};
+}
+
 // ---------------------------------------------------------------------------
-// Regular pairs. One thread per test cell; the block's trial cells are walked
-// in order with wave-uniform (scalar) loads of the trial geometry. P0P0:
-// both spaces P0, the thread owns output row i and writes column j directly
-// (coalesced along the test index). Otherwise the local block is added with
-// atomics at the DOF indices.
+// Regular pairs with a P1 space on either side. One thread per test cell; the
+// block's trial cells are walked in order with wave-uniform (scalar) loads of
+// the trial geometry; the local block is added with atomics at the DOF indices.
 // ---------------------------------------------------------------------------
-template <int Q, int OP, bool HELM, bool P0P0>
+template <int Q, int OP, bool HELM, bool CENTRED>
 __global__ void __launch_bounds__(256) regular_kernel(const RegularArgs A) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= A.n_test) return;
+  const bool i_ok = i < A.n_test;
+  const int il = i_ok ? i : A.n_test - 1;  // clamped: loads stay in bounds
   double xp[Q][3], wp[Q];
 #pragma unroll
   for (int p = 0; p < Q; ++p) {
-    const double* s = A.test_pts + ((size_t)i * Q + p) * 4;
+    const double* s = A.test_pts + ((size_t)il * Q + p) * 4;
     xp[p][0] = s[0];
     xp[p][1] = s[1];
     xp[p][2] = s[2];
     wp[p] = s[3];
   }
-  const double* tgeo = A.test_geo + (size_t)i * kGeoStride;
+  const double* tgeo = A.test_geo + (size_t)il * kGeoStride;
   const double nx[3] = {tgeo[9], tgeo[10], tgeo[11]};
-  const int tv0 = A.test_vid[4 * i], tv1 = A.test_vid[4 * i + 1], tv2 = A.test_vid[4 * i + 2];
+  const double ct[3] = {tgeo[kGeoCentroid], tgeo[kGeoCentroid + 1], tgeo[kGeoCentroid + 2]};
+  const int tv0 = A.test_vid[4 * il], tv1 = A.test_vid[4 * il + 1], tv2 = A.test_vid[4 * il + 2];
+  const auto trial_pts = constant_view(A.trial_pts);
+  const auto trial_geo = constant_view(A.trial_geo);
+  const auto trial_vid = constant_view(A.trial_vid);
   const int j0 = blockIdx.y * A.trial_chunk;
   const int j1 = min(j0 + A.trial_chunk, A.n_trial);
   for (int j = j0; j < j1; ++j) {
-    const double* sp = A.trial_pts + (size_t)j * Q * 4;
-    const double* sgeo = A.trial_geo + (size_t)j * kGeoStride;
-    const int sv0 = A.trial_vid[4 * j], sv1 = A.trial_vid[4 * j + 1], sv2 = A.trial_vid[4 * j + 2];
+    const auto spp = trial_pts + (size_t)j * Q * 4;
+    double sp[Q * 4];
+#pragma unroll
+    for (int m = 0; m < Q * 4; ++m) sp[m] = spp[m];
+    const auto sg = trial_geo + (size_t)j * kGeoStride;
+    double sgeo[kGeoStride];
+#pragma unroll
+    for (int m = 0; m < kGeoStride; ++m) sgeo[m] = sg[m];
+    const int sv0 = trial_vid[4 * j], sv1 = trial_vid[4 * j + 1], sv2 = trial_vid[4 * j + 2];
     const bool adjacent = A.same_grid && (tv0 == sv0 || tv0 == sv1 || tv0 == sv2 || tv1 == sv0 || tv1 == sv1 ||
                                           tv1 == sv2 || tv2 == sv0 || tv2 == sv1 || tv2 == sv2);
+    if (adjacent || !i_ok) continue;
     const double ny[3] = {sgeo[9], sgeo[10], sgeo[11]};
-    if constexpr (P0P0) {
+    double kR = 0.0;
+    cplx e = {1.0, 0.0};
+    if constexpr (CENTRED) e = pair_phase(ct, sgeo + kGeoCentroid, A.k, &kR);
+    cplx M[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) M[a][b] = {0.0, 0.0};
+#pragma unroll
+    for (int p = 0; p < Q; ++p) {
+      cplx T[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const cplx kv = kernel_scalar<OP, HELM, CENTRED>(xp[p][0] - sp[4 * q], xp[p][1] - sp[4 * q + 1],
+                                                         xp[p][2] - sp[4 * q + 2], A.k, nx, ny, kR);
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          const double f = (A.nbs == 3 ? A.phi[q][b] : (b == 0 ? 1.0 : 0.0)) * sp[4 * q + 3];
+          T[b].re = fma(kv.re, f, T[b].re);
+          T[b].im = fma(kv.im, f, T[b].im);
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const double f = (A.nbt == 3 ? A.phi[p][a] : (a == 0 ? 1.0 : 0.0)) * wp[p];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          M[a][b].re = fma(f, T[b].re, M[a][b].re);
+          M[a][b].im = fma(f, T[b].im, M[a][b].im);
+        }
+      }
+    }
+    cplx msum = {0.0, 0.0};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        if constexpr (CENTRED) M[a][b] = cmul(M[a][b], e);
+        msum.re += M[a][b].re;
+        msum.im += M[a][b].im;
+      }
+    for (int b = 0; b < A.nbs; ++b) {
+      const long long col = A.trial_dofs[(size_t)j * A.nbs + b];
+      for (int a = 0; a < A.nbt; ++a) {
+        const long long row = A.test_dofs[(size_t)i * A.nbt + a];
+        const cplx v = local_entry<OP, HELM>(M, a, b, msum, A.nbt, A.nbs, tgeo, sgeo, a, b, A.k);
+        store_value(A.out, row + A.ld * col, v, A.out_type, kAtomic);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// P0 x P0 regular pairs, tiled. One workgroup (4 wavefronts) per 64 x 64 tile
+// (tile row a: test cells 64a + lane; tile column b: trial cells, 16 per
+// wavefront in two chunks of 8). The tile is written column-major with
+// coalesced stores along the test index. When `symmetric` is set (single
+// layer / hypersingular with identical test and trial space), only tiles with
+// a <= b are launched and an off-diagonal tile is also written transposed at
+// (b, a): A[j][i] sums the same quadrature terms as A[i][j] because G is
+// symmetric and both cells use the same rule. The transpose goes through LDS
+// so the mirrored stores are 128-byte contiguous segments.
+// ---------------------------------------------------------------------------
+constexpr int kTile = 64;
+constexpr int kChunk = 8;
+
+template <int Q, int OP, bool HELM, bool CENTRED>
+__device__ __forceinline__ cplx p0_pair(const double (&xp)[Q][3], const double (&wp)[Q], const double* nx,
+                                        const double (&sp)[Q * 4], const double* ny, double k, double kR) {
+  cplx acc = {0.0, 0.0};
+#pragma unroll
+  for (int p = 0; p < Q; ++p) {
+    cplx ap = {0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const cplx kv = kernel_scalar<OP, HELM, CENTRED>(xp[p][0] - sp[4 * q], xp[p][1] - sp[4 * q + 1],
+                                                       xp[p][2] - sp[4 * q + 2], k, nx, ny, kR);
+      ap.re = fma(kv.re, sp[4 * q + 3], ap.re);
+      ap.im = fma(kv.im, sp[4 * q + 3], ap.im);
+    }
+    acc.re = fma(ap.re, wp[p], acc.re);
+    acc.im = fma(ap.im, wp[p], acc.im);
+  }
+  return acc;
+}
+
+template <int Q, int OP, bool HELM, bool CENTRED>
+__global__ void __launch_bounds__(256) regular_p0_tiled(const RegularArgs A, const int2* __restrict__ tiles,
+                                                        int symmetric) {
+  __shared__ double2 lds[4][kChunk][kTile];
+  const int2 tile = tiles[blockIdx.x];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: lets trial loads be scalar
+  const int i = tile.x * kTile + lane;
+  const bool i_ok = i < A.n_test;
+  const int il = i_ok ? i : A.n_test - 1;  // clamped: loads stay in bounds
+  double xp[Q][3], wp[Q];
+#pragma unroll
+  for (int p = 0; p < Q; ++p) {
+    const double* s = A.test_pts + ((size_t)il * Q + p) * 4;
+    xp[p][0] = s[0];
+    xp[p][1] = s[1];
+    xp[p][2] = s[2];
+    wp[p] = s[3];
+  }
+  const double* tgeo = A.test_geo + (size_t)il * kGeoStride;
+  const double nx[3] = {tgeo[9], tgeo[10], tgeo[11]};
+  const double ct[3] = {tgeo[kGeoCentroid], tgeo[kGeoCentroid + 1], tgeo[kGeoCentroid + 2]};
+  const int tv0 = A.test_vid[4 * il], tv1 = A.test_vid[4 * il + 1], tv2 = A.test_vid[4 * il + 2];
+  const bool mirror = symmetric && tile.x != tile.y;
+  const auto trial_pts = constant_view(A.trial_pts);
+  const auto trial_geo = constant_view(A.trial_geo);
+  const auto trial_vid = constant_view(A.trial_vid);
+  for (int c = 0; c < 2; ++c) {
+    const int jbase = tile.y * kTile + wave * 16 + c * kChunk;
+#pragma unroll 1
+    for (int jj = 0; jj < kChunk; ++jj) {
+      const int j = jbase + jj;
+      if (j >= A.n_trial) {  // wave-uniform
+        lds[wave][jj][lane] = make_double2(0.0, 0.0);
+        continue;
+      }
+      // wave-uniform trial cell: loaded unconditionally (scalar loads) before any per-lane branch
+      const auto spp = trial_pts + (size_t)j * Q * 4;
+      double sp[Q * 4];
+#pragma unroll
+      for (int m = 0; m < Q * 4; ++m) sp[m] = spp[m];
+      const auto sg = trial_geo + (size_t)j * kGeoStride;
+      const int sv0 = trial_vid[4 * j], sv1 = trial_vid[4 * j + 1], sv2 = trial_vid[4 * j + 2];
+      const bool adjacent = A.same_grid && (tv0 == sv0 || tv0 == sv1 || tv0 == sv2 || tv1 == sv0 || tv1 == sv1 ||
+                                            tv1 == sv2 || tv2 == sv0 || tv2 == sv1 || tv2 == sv2);
+      const double ny[3] = {sg[9], sg[10], sg[11]};
       cplx acc = {0.0, 0.0};
       if (!adjacent) {
-#pragma unroll
-        for (int p = 0; p < Q; ++p) {
-          cplx ap = {0.0, 0.0};
-#pragma unroll
-          for (int q = 0; q < Q; ++q) {
-            const cplx kv = kernel_scalar<OP, HELM>(xp[p][0] - sp[4 * q], xp[p][1] - sp[4 * q + 1],
-                                                     xp[p][2] - sp[4 * q + 2], A.k, nx, ny);
-            ap.re += kv.re * sp[4 * q + 3];
-            ap.im += kv.im * sp[4 * q + 3];
-          }
-          acc.re += ap.re * wp[p];
-          acc.im += ap.im * wp[p];
+        if constexpr (CENTRED) {
+          const double cs[3] = {sg[kGeoCentroid], sg[kGeoCentroid + 1], sg[kGeoCentroid + 2]};
+          double kR;
+          const cplx e = pair_phase(ct, cs, A.k, &kR);
+          acc = cmul(p0_pair<Q, OP, HELM, true>(xp, wp, nx, sp, ny, A.k, kR), e);
+        } else {
+          acc = p0_pair<Q, OP, HELM, false>(xp, wp, nx, sp, ny, A.k, 0.0);
         }
       }
-      if (adjacent && A.mode == kAccumulate) continue;
-      cplx M[3][3];
-      M[0][0] = acc;
-      const cplx v = local_entry<OP, HELM>(M, 0, 0, acc, 1, 1, tgeo, sgeo, 0, 0, A.k);
-      store_value(A.out, (long long)i + A.ld * (long long)j, v, A.out_type, A.mode);
-    } else {
-      if (adjacent) continue;
-      cplx M[3][3];
-#pragma unroll
-      for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b) M[a][b] = {0.0, 0.0};
-#pragma unroll
-      for (int p = 0; p < Q; ++p) {
-        cplx T[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
-#pragma unroll
-        for (int q = 0; q < Q; ++q) {
-          const cplx kv = kernel_scalar<OP, HELM>(xp[p][0] - sp[4 * q], xp[p][1] - sp[4 * q + 1],
-                                                   xp[p][2] - sp[4 * q + 2], A.k, nx, ny);
-#pragma unroll
-          for (int b = 0; b < 3; ++b) {
-            const double f = (A.nbs == 3 ? A.phi[q][b] : (b == 0 ? 1.0 : 0.0)) * sp[4 * q + 3];
-            T[b].re += kv.re * f;
-            T[b].im += kv.im * f;
-          }
-        }
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          const double f = (A.nbt == 3 ? A.phi[p][a] : (a == 0 ? 1.0 : 0.0)) * wp[p];
-#pragma unroll
-          for (int b = 0; b < 3; ++b) {
-            M[a][b].re += f * T[b].re;
-            M[a][b].im += f * T[b].im;
-          }
-        }
+      if constexpr (OP == 3) {  // P0 hypersingular: only the Helmholtz -k^2 n_x.n_y term survives
+        const double nn = HELM ? -A.k * A.k * (nx[0] * ny[0] + nx[1] * ny[1] + nx[2] * ny[2]) : 0.0;
+        acc = {nn * acc.re, nn * acc.im};
       }
-      cplx msum = {0.0, 0.0};
-      for (int a = 0; a < A.nbt; ++a)
-        for (int b = 0; b < A.nbs; ++b) {
-          msum.re += M[a][b].re;
-          msum.im += M[a][b].im;
-        }
-      for (int b = 0; b < A.nbs; ++b) {
-        const long long col = A.trial_dofs[(size_t)j * A.nbs + b];
-        for (int a = 0; a < A.nbt; ++a) {
-          const long long row = A.test_dofs[(size_t)i * A.nbt + a];
-          const cplx v = local_entry<OP, HELM>(M, a, b, msum, A.nbt, A.nbs, tgeo, sgeo, a, b, A.k);
-          store_value(A.out, row + A.ld * col, v, A.out_type, kAtomic);
-        }
+      // adjacent pairs get 0 here (their Duffy value is added by the singular kernel)
+      const cplx v = {acc.re * kInv4Pi, acc.im * kInv4Pi};
+      if (i_ok) store_value(A.out, (long long)i + A.ld * (long long)j, v, A.out_type, A.mode);
+      if (mirror) lds[wave][jj][lane] = make_double2(v.re, v.im);
+    }
+    if (mirror) {  // block-uniform
+      __syncthreads();
+      const int row = lane & (kChunk - 1);
+      const int jr = jbase + row;  // row of the mirrored entry (a trial cell of this tile)
+#pragma unroll
+      for (int s = 0; s < kTile / kChunk; ++s) {
+        const int col = s * kChunk + (lane >> 3);
+        const int ic = tile.x * kTile + col;  // column of the mirrored entry (a test cell)
+        const double2 val = lds[wave][row][col];
+        if (jr < A.n_trial && ic < A.n_test)
+          store_value(A.out, (long long)jr + A.ld * (long long)ic, {val.x, val.y}, A.out_type, A.mode);
       }
+      __syncthreads();
     }
   }
 }
@@ -287,90 +431,102 @@ __device__ __forceinline__ double wave_sum(double v) {
 // The canonical rule (shared vertices at canonical positions) is mapped to each
 // cell by permuting its vertices; normals, Jacobian determinants and curls come
 // from the cell's own vertex order, and canonical basis function k of a P1 cell
-// is its local basis function perm[k].
+// is its local basis function perm[k]. P0P0: both spaces P0 (one accumulator).
+// SMALL (Helmholtz): |k| |x - y| <= 0.5 for every point pair of every adjacent
+// cell pair, so e^{ikr} uses the small-argument series without reduction.
 // ---------------------------------------------------------------------------
-template <int OP, bool HELM>
+__device__ __forceinline__ int perm_digit(int code, int k) { return (code >> (2 * k)) & 3; }
+
+template <int OP, bool HELM, bool P0P0, bool SMALL>
 __global__ void __launch_bounds__(256) singular_kernel(const SingularArgs A) {
-  const int pair = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
+  const int pair = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6));
   const int lane = threadIdx.x & 63;
   if (pair >= A.npairs) return;  // uniform per wavefront
-  const int4 P = A.pairs[pair];
-  const int tc = P.x, sc = P.y, m = P.z, code = P.w;
-  int tperm[3], sperm[3];
-#pragma unroll
-  for (int kk = 0; kk < 3; ++kk) {
-    tperm[kk] = (code >> (2 * kk)) & 3;
-    sperm[kk] = (code >> (8 + 2 * kk)) & 3;
-  }
-  const double* tgeo = A.test_geo + (size_t)tc * kGeoStride;
-  const double* sgeo = A.trial_geo + (size_t)sc * kGeoStride;
+  const auto pw = constant_view((const int*)A.pairs) + 4 * (size_t)pair;
+  const int tc = pw[0], sc = pw[1], m = pw[2], code = pw[3];
+  const auto tg = constant_view(A.test_geo) + (size_t)tc * kGeoStride;
+  const auto sg = constant_view(A.trial_geo) + (size_t)sc * kGeoStride;
   double tv[3][3], sv[3][3];
 #pragma unroll
-  for (int kk = 0; kk < 3; ++kk)
+  for (int kk = 0; kk < 3; ++kk) {
+    const int pt = perm_digit(code, kk), ps = perm_digit(code >> 8, kk);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      tv[kk][d] = tgeo[3 * tperm[kk] + d];
-      sv[kk][d] = sgeo[3 * sperm[kk] + d];
+      tv[kk][d] = tg[3 * pt + d];
+      sv[kk][d] = sg[3 * ps + d];
     }
-  const double nx[3] = {tgeo[9], tgeo[10], tgeo[11]};
-  const double ny[3] = {sgeo[9], sgeo[10], sgeo[11]};
-  const double jj = tgeo[12] * sgeo[12];
+  }
+  const double nx[3] = {tg[9], tg[10], tg[11]};
+  const double ny[3] = {sg[9], sg[10], sg[11]};
+  const double jj = tg[12] * sg[12];
   const int off = A.rule_off[m], n = A.rule_n[m], tot = A.rule_total;
-  cplx M[3][3];
+  constexpr int NB = P0P0 ? 1 : 3;
+  cplx M[NB][NB];
 #pragma unroll
-  for (int a = 0; a < 3; ++a)
+  for (int a = 0; a < NB; ++a)
 #pragma unroll
-    for (int b = 0; b < 3; ++b) M[a][b] = {0.0, 0.0};
+    for (int b = 0; b < NB; ++b) M[a][b] = {0.0, 0.0};
   for (int pt = lane; pt < n; pt += 64) {
     const double a0 = A.rule[off + pt], a1 = A.rule[tot + off + pt];
     const double b0 = A.rule[2 * tot + off + pt], b1 = A.rule[3 * tot + off + pt];
     const double w = A.rule[4 * tot + off + pt] * jj;
     const double lt[3] = {1.0 - a0 - a1, a0, a1};
     const double ls[3] = {1.0 - b0 - b1, b0, b1};
-    double x[3], y[3];
+    double d[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      x[d] = tv[0][d] * lt[0] + tv[1][d] * lt[1] + tv[2][d] * lt[2];
-      y[d] = sv[0][d] * ls[0] + sv[1][d] * ls[1] + sv[2][d] * ls[2];
-    }
-    cplx kv = kernel_scalar<OP, HELM>(x[0] - y[0], x[1] - y[1], x[2] - y[2], A.k, nx, ny);
+    for (int c = 0; c < 3; ++c)
+      d[c] = (tv[0][c] * lt[0] + tv[1][c] * lt[1] + tv[2][c] * lt[2]) -
+             (sv[0][c] * ls[0] + sv[1][c] * ls[1] + sv[2][c] * ls[2]);
+    cplx kv = kernel_scalar<OP, HELM, SMALL>(d[0], d[1], d[2], A.k, nx, ny, 0.0);
     kv.re *= w;
     kv.im *= w;
+    if constexpr (P0P0) {
+      M[0][0].re += kv.re;
+      M[0][0].im += kv.im;
+    } else {
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const double fa = A.nbt == 3 ? lt[a] : (a == 0 ? 1.0 : 0.0);
+      for (int a = 0; a < 3; ++a) {
+        const double fa = A.nbt == 3 ? lt[a] : (a == 0 ? 1.0 : 0.0);
 #pragma unroll
-      for (int b = 0; b < 3; ++b) {
-        const double f = fa * (A.nbs == 3 ? ls[b] : (b == 0 ? 1.0 : 0.0));
-        M[a][b].re += kv.re * f;
-        M[a][b].im += kv.im * f;
+        for (int b = 0; b < 3; ++b) {
+          const double f = fa * (A.nbs == 3 ? ls[b] : (b == 0 ? 1.0 : 0.0));
+          M[a][b].re = fma(kv.re, f, M[a][b].re);
+          M[a][b].im = fma(kv.im, f, M[a][b].im);
+        }
       }
     }
   }
+  cplx Ml[3][3];
   cplx msum = {0.0, 0.0};
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
-    for (int b = 0; b < 3; ++b) {
-      M[a][b].re = wave_sum(M[a][b].re);
-      M[a][b].im = HELM ? wave_sum(M[a][b].im) : 0.0;
-      msum.re += M[a][b].re;
-      msum.im += M[a][b].im;
+    for (int b = 0; b < 3; ++b) Ml[a][b] = {0.0, 0.0};
+#pragma unroll
+  for (int a = 0; a < NB; ++a)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      Ml[a][b].re = wave_sum(M[a][b].re);
+      Ml[a][b].im = HELM ? wave_sum(M[a][b].im) : 0.0;
+      msum.re += Ml[a][b].re;
+      msum.im += Ml[a][b].im;
     }
   if (lane != 0) return;
-  for (int b = 0; b < A.nbs; ++b) {
-    const int lb = A.nbs == 3 ? sperm[b] : 0;
-    for (int a = 0; a < A.nbt; ++a) {
-      const int la = A.nbt == 3 ? tperm[a] : 0;
-      const cplx v = local_entry<OP, HELM>(M, a, b, msum, A.nbt, A.nbs, tgeo, sgeo, la, lb, A.k);
+  const int nbt = P0P0 ? 1 : A.nbt, nbs = P0P0 ? 1 : A.nbs;
+  for (int b = 0; b < nbs; ++b) {
+    const int lb = nbs == 3 ? perm_digit(code >> 8, b) : 0;
+    for (int a = 0; a < nbt; ++a) {
+      const int la = nbt == 3 ? perm_digit(code, a) : 0;
+      const cplx v = local_entry<OP, HELM>(Ml, a, b, msum, nbt, nbs, A.test_geo + (size_t)tc * kGeoStride,
+                                           A.trial_geo + (size_t)sc * kGeoStride, la, lb, A.k);
       if (A.mode == kCoo) {
-        const long long e = (long long)pair * A.nbt * A.nbs + la + A.nbt * lb;
+        const long long e = (long long)pair * nbt * nbs + la + nbt * lb;
         double* o = (double*)A.out;
         o[2 * e] = v.re;
         o[2 * e + 1] = v.im;
       } else {
-        const long long row = A.test_dofs[(size_t)tc * A.nbt + la];
-        const long long col = A.trial_dofs[(size_t)sc * A.nbs + lb];
+        const long long row = A.test_dofs[(size_t)tc * nbt + la];
+        const long long col = A.trial_dofs[(size_t)sc * nbs + lb];
         store_value(A.out, row + A.ld * col, v, A.out_type, A.mode);
       }
     }

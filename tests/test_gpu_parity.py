@@ -74,6 +74,21 @@ def test_helmholtz_sl_p0_icosphere_l3():
     _check("helmholtz", "single_layer", icosphere(3), "dp0", "dp0", k=5.0)
 
 
+@pytest.mark.parametrize("op", list(OPS))
+@pytest.mark.parametrize("space", ["dp0", "p1"])
+def test_helmholtz_centred_phase(op, space):
+    """k = 2.5 on the level-3 icosphere: |k| (rad_i + rad_j) <= 0.43, so the regular kernels
+    take the phase relative to each cell pair's centroid distance (small-argument sin/cos)."""
+    _check("helmholtz", op, icosphere(3), space, space, k=2.5)
+
+
+@pytest.mark.parametrize("op,space", [("single_layer", "dp0"), ("double_layer", "p1"), ("hypersingular", "p1")])
+def test_helmholtz_small_phase_adjacent(op, space):
+    """k = 2.5 on the level-4 icosphere (5120 cells): |k| |x - y| <= 0.43 on every adjacent
+    pair, so the Duffy kernel also uses the small-argument sin/cos."""
+    _check("helmholtz", op, icosphere(4), space, space, k=2.5)
+
+
 @pytest.mark.parametrize("test,trial", [("dp0", "p1"), ("p1", "dp0"), ("dp1", "dp1"), ("dp1", "p1")])
 @pytest.mark.parametrize("op", ["single_layer", "double_layer", "hypersingular"])
 def test_mixed_spaces(test, trial, op):
