@@ -70,6 +70,7 @@ _SIGNATURES = {
     "matrix_csr_indices_copy": (None, [c_vp, c_vp]),
     "matrix_csr_indptr_copy": (None, [c_vp, c_vp]),
     "matrix_t_free": (None, [c_vp]),
+    "boundary_assembler_apply": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_int]),
     "boundary_assembler_set_profiling": (c_int, [c_vp, c_int]),
     "boundary_assembler_kernel_times": (c_int, [c_vp, c_vp]),
     "bempp_triangle_rule": (c_int, [c_int, c_vp, c_vp]),

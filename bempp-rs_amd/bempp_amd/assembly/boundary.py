@@ -224,6 +224,18 @@ class BoundaryAssembler:
                                                                       test_space._rs_space))
         return _convert_to_scipy(m, self.dtype)
 
+    def apply(self, trial_space, test_space, u, transpose=False):
+        """y = A u (A^T u with transpose) without assembling A (P0 spaces): the operator is
+        recomputed on the GPU."""
+        self._check_spaces(trial_space, test_space)
+        u = np.ascontiguousarray(u, dtype=self.dtype)
+        if u.shape != (trial_space.global_size,):
+            raise ValueError("u must have n_trial entries")
+        y = np.zeros(test_space.global_size, dtype=self.dtype)
+        L.check(L.lib().boundary_assembler_apply(self._rs_assembler, trial_space._rs_space, test_space._rs_space,
+                                                  L.ptr(u), L.ptr(y), 1 if transpose else 0))
+        return y
+
     def shape_gradient(self, trial_space, test_space, u, v):
         """d(v^T A(X) u)/dX for the grid's vertex coordinates X, shape [npoints, 3]
         (real for Laplace, complex for Helmholtz). Matrix-free on the GPU."""

@@ -355,6 +355,19 @@ int boundary_assembler_shape_gradient(const BoundaryAssemblerT* a, const SpaceT*
       1);
 }
 
+int boundary_assembler_apply(const BoundaryAssemblerT* a, const SpaceT* trial, const SpaceT* test, const void* u,
+                              void* y, int transpose) {
+  return guarded(
+      [&]() {
+        check_spaces(a, trial, test);
+        need(u, "u");
+        need(y, "y");
+        apply(*a->a, *trial->s, *test->s, u, y, transpose != 0);
+        return 0;
+      },
+      1);
+}
+
 int boundary_assembler_set_profiling(BoundaryAssemblerT* a, int enable) {
   return guarded(
       [&]() {

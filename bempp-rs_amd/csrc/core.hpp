@@ -147,6 +147,9 @@ void assemble_host(const Assembler& a, const FunctionSpace& trial, const Functio
 // in the assembler dtype (interleaved for complex).
 void assemble_singular_coo(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test,
                            std::vector<int64_t>& rows, std::vector<int64_t>& cols, std::vector<double>& vals);
+// y = A u (or A^T u) without assembling A (P0 x P0 spaces); u, y in the assembler dtype.
+void apply(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test, const void* u, void* y,
+           bool transpose = false);
 // Shape gradient of v^T A(X) u with respect to the grid's vertex coordinates.
 void shape_gradient(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test, const void* u,
                     const void* v, void* grad);

@@ -12,6 +12,7 @@
 
 #include "core.hpp"
 #include "kernels.hpp"
+#include "kernels_mixed.hpp"
 
 namespace bempp_amd {
 
@@ -33,6 +34,9 @@ struct DeviceGridCache {
   int npairs = -1;
   std::vector<AdjacentPair> host_pairs;
   double max_radius = 0.0;        // largest centroid-to-vertex distance of any cell
+  std::map<int, float*> pts32;    // regular point count -> [nc][Q][4] FP32 offsets from the centroid, w*jdet
+  int2* near = nullptr;           // non-adjacent pairs closer than the mixed-precision far bound (superset)
+  int nnear = -1;
 };
 
 struct DeviceSpaceCache {
@@ -47,6 +51,8 @@ void free_device_grid_cache(DeviceGridCache* c) {
     (void)hipFree(c->geo);
     (void)hipFree(c->vid);
     for (auto& kv : c->pts) (void)hipFree(kv.second);
+    for (auto& kv : c->pts32) (void)hipFree(kv.second);
+    (void)hipFree(c->near);
     (void)hipFree(c->pairs);
     (void)hipSetDevice(cur);
   }
@@ -177,6 +183,108 @@ const double* grid_points_for_rule(Grid& g, DeviceGridCache* c, int Q, const std
   double* d = upload(pts);
   c->pts[Q] = d;
   return d;
+}
+
+// FP32 copies of the regular points as offsets from the cell centroid (mixed precision).
+const float* grid_points32_for_rule(Grid& g, DeviceGridCache* c, int Q, const std::vector<double>& xi,
+                                    const std::vector<double>& w) {
+  std::lock_guard<std::mutex> lk(g.cache_mutex);
+  auto it = c->pts32.find(Q);
+  if (it != c->pts32.end()) return it->second;
+  const int64_t nc = g.nc();
+  std::vector<float> p32(nc * Q * 4);
+  for (int64_t i = 0; i < nc; ++i) {
+    const double* v0 = &g.points[3 * g.cells[3 * i]];
+    const double* v1 = &g.points[3 * g.cells[3 * i + 1]];
+    const double* v2 = &g.points[3 * g.cells[3 * i + 2]];
+    double J0[3], J1[3], cen[3];
+    for (int d = 0; d < 3; ++d) {
+      J0[d] = v1[d] - v0[d];
+      J1[d] = v2[d] - v0[d];
+      cen[d] = (v0[d] + v1[d] + v2[d]) / 3.0;
+    }
+    const double cr[3] = {J0[1] * J1[2] - J0[2] * J1[1], J0[2] * J1[0] - J0[0] * J1[2],
+                          J0[0] * J1[1] - J0[1] * J1[0]};
+    const double jdet = std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+    for (int p = 0; p < Q; ++p) {
+      float* o = &p32[(i * Q + p) * 4];
+      for (int d = 0; d < 3; ++d) o[d] = (float)((v0[d] + J0[d] * xi[2 * p] + J1[d] * xi[2 * p + 1]) - cen[d]);
+      o[3] = (float)(w[p] * jdet);
+    }
+  }
+  float* d = upload(p32);
+  c->pts32[Q] = d;
+  return d;
+}
+
+// Non-adjacent cell pairs with centroid distance below 1.01 x the far bound
+// kNearFactor (rad_i + rad_j), found with a uniform bucket grid over centroids.
+// A superset of the pairs the mixed tile kernel leaves out; the near-pair kernel
+// re-applies the exact device predicate.
+std::vector<int2> near_pairs_host(const Grid& g) {
+  const int64_t nc = g.nc();
+  std::vector<double> cen(3 * nc), rad(nc);
+  double rmax = 0.0, lo[3] = {1e300, 1e300, 1e300};
+  for (int64_t i = 0; i < nc; ++i) {
+    double r2 = 0.0;
+    for (int d = 0; d < 3; ++d)
+      cen[3 * i + d] = (g.points[3 * g.cells[3 * i] + d] + g.points[3 * g.cells[3 * i + 1] + d] +
+                        g.points[3 * g.cells[3 * i + 2] + d]) / 3.0;
+    for (int m = 0; m < 3; ++m) {
+      double s = 0.0;
+      for (int d = 0; d < 3; ++d) {
+        const double t = g.points[3 * g.cells[3 * i + m] + d] - cen[3 * i + d];
+        s += t * t;
+      }
+      r2 = std::max(r2, s);
+    }
+    rad[i] = std::sqrt(r2) * (1.0 + 1e-12);
+    rmax = std::max(rmax, rad[i]);
+    for (int d = 0; d < 3; ++d) lo[d] = std::min(lo[d], cen[3 * i + d]);
+  }
+  const double factor = dev::kNearFactor * 1.01;
+  const double h = std::max(factor * 2.0 * rmax, 1e-300);
+  std::map<std::tuple<int64_t, int64_t, int64_t>, std::vector<int64_t>> buckets;
+  auto key = [&](int64_t i) {
+    return std::make_tuple((int64_t)std::floor((cen[3 * i] - lo[0]) / h), (int64_t)std::floor((cen[3 * i + 1] - lo[1]) / h),
+                           (int64_t)std::floor((cen[3 * i + 2] - lo[2]) / h));
+  };
+  for (int64_t i = 0; i < nc; ++i) buckets[key(i)].push_back(i);
+  std::vector<int2> out;
+  for (int64_t i = 0; i < nc; ++i) {
+    const auto k = key(i);
+    const int64_t* tv = &g.cells[3 * i];
+    for (int dx = -1; dx <= 1; ++dx)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dz = -1; dz <= 1; ++dz) {
+          auto it = buckets.find(std::make_tuple(std::get<0>(k) + dx, std::get<1>(k) + dy, std::get<2>(k) + dz));
+          if (it == buckets.end()) continue;
+          for (int64_t j : it->second) {
+            const int64_t* sv = &g.cells[3 * j];
+            bool adj = false;
+            for (int a = 0; a < 3 && !adj; ++a)
+              for (int b = 0; b < 3; ++b)
+                if (tv[a] == sv[b]) { adj = true; break; }
+            if (adj) continue;
+            double R2 = 0.0;
+            for (int d = 0; d < 3; ++d) R2 += (cen[3 * i + d] - cen[3 * j + d]) * (cen[3 * i + d] - cen[3 * j + d]);
+            const double lim = factor * (rad[i] + rad[j]);
+            if (R2 < lim * lim) out.push_back(make_int2((int)i, (int)j));
+          }
+        }
+  }
+  return out;
+}
+
+const int2* grid_near_pairs(Grid& g, DeviceGridCache* c, int* n) {
+  std::lock_guard<std::mutex> lk(g.cache_mutex);
+  if (c->nnear < 0) {
+    std::vector<int2> v = near_pairs_host(g);
+    c->near = upload(v);
+    c->nnear = (int)v.size();
+  }
+  *n = c->nnear;
+  return c->near;
 }
 
 const int4* grid_pairs(Grid& g, DeviceGridCache* c, int* npairs) {
@@ -340,6 +448,44 @@ void launch_regular(int op, bool helm, bool centred, bool tiled, int Q, dim3 gri
   HIP_CHECK(hipGetLastError());
 }
 
+// Mixed precision (P0 x P0, one grid): FP32 far-pair tiles, then the FP64 near-pair list.
+template <int OP, bool HELM, bool CENTRED>
+void launch_mixed_q(int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& A, const int2* tiles, int sym,
+                    const int2* near, int nnear) {
+  dim3 ng((unsigned)((nnear + 255) / 256));
+#define BEMPP_LAUNCH(QQ)                                                                      \
+  dev::regular_p0_tiled_mixed<QQ, OP, HELM><<<grid, 256, 0, s>>>(A, tiles, sym);              \
+  if (nnear > 0) dev::near_pair_kernel<QQ, OP, HELM, CENTRED><<<ng, 256, 0, s>>>(A, near, nnear);
+  switch (Q) {
+    case 1: BEMPP_LAUNCH(1); break;
+    case 3: BEMPP_LAUNCH(3); break;
+    case 6: BEMPP_LAUNCH(6); break;
+    case 7: BEMPP_LAUNCH(7); break;
+    default: throw Error("unsupported regular quadrature point count");
+  }
+#undef BEMPP_LAUNCH
+}
+
+template <bool HELM, bool CENTRED>
+void launch_mixed_op(int op, int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& A, const int2* tiles, int sym,
+                     const int2* near, int nnear) {
+  switch (op) {
+    case kSL: launch_mixed_q<kSL, HELM, CENTRED>(Q, grid, s, A, tiles, sym, near, nnear); break;
+    case kDL: launch_mixed_q<kDL, HELM, CENTRED>(Q, grid, s, A, tiles, sym, near, nnear); break;
+    case kADL: launch_mixed_q<kADL, HELM, CENTRED>(Q, grid, s, A, tiles, sym, near, nnear); break;
+    case kHS: launch_mixed_q<kHS, HELM, CENTRED>(Q, grid, s, A, tiles, sym, near, nnear); break;
+    default: throw Error("operator not supported by the dense assembler");
+  }
+}
+
+void launch_mixed(int op, bool helm, bool centred, int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& A,
+                  const int2* tiles, int sym, const int2* near, int nnear) {
+  if (!helm) launch_mixed_op<false, false>(op, Q, grid, s, A, tiles, sym, near, nnear);
+  else if (centred) launch_mixed_op<true, true>(op, Q, grid, s, A, tiles, sym, near, nnear);
+  else throw Error("mixed precision needs the centred-phase bound |k| (rad_i + rad_j) <= 0.5");
+  HIP_CHECK(hipGetLastError());
+}
+
 template <bool HELM, bool P0P0, bool SMALL>
 void launch_singular_t(int op, dim3 grid, hipStream_t s, const dev::SingularArgs& A) {
   switch (op) {
@@ -370,6 +516,9 @@ struct Prepared {
   int device;
   bool centred;         // Helmholtz phases taken relative to the cell-pair centroid distance
   bool small_adjacent;  // |k| |x - y| <= 0.5 on every adjacent cell pair
+  bool mixed;           // FP32 far pairs + FP64 near-pair list (P0 x P0 on one grid)
+  const int2* near;
+  int nnear;
   hipStream_t stream;
   int Q;
   bool same_grid;
@@ -427,6 +576,16 @@ Prepared prepare(const Assembler& a, const FunctionSpace& trial_c, const Functio
   dev::RegularArgs& R = P.R;
   R.test_pts = grid_points_for_rule(tgrid, tc, P.Q, xi, w);
   R.trial_pts = grid_points_for_rule(sgrid, sc, P.Q, xi, w);
+  // mixed precision: P0 x P0 on one grid, and for Helmholtz the centred phase bound
+  P.mixed = a.options.precision == kMixed && test.nb == 1 && trial.nb == 1 && P.same_grid &&
+            (a.kind == kLaplace || P.centred);
+  P.near = nullptr;
+  P.nnear = 0;
+  R.test_pts32 = R.trial_pts32 = nullptr;
+  if (P.mixed) {
+    R.test_pts32 = R.trial_pts32 = grid_points32_for_rule(tgrid, tc, P.Q, xi, w);
+    P.near = grid_near_pairs(tgrid, tc, &P.nnear);
+  }
   R.test_geo = tc->geo;
   R.trial_geo = sc->geo;
   R.test_vid = tc->vid;
@@ -478,6 +637,27 @@ Prepared prepare(const Assembler& a, const FunctionSpace& trial_c, const Functio
 
 }  // namespace
 
+
+// Geometry, rule tables and adjacent pairs for the matrix-free kernels (gradient.hip).
+GradientInputs gradient_inputs(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test) {
+  Prepared P = prepare(a, trial, test, nullptr, nullptr);
+  GradientInputs g{};
+  g.R = P.R;
+  g.pairs = P.S.pairs;
+  g.npairs = P.npairs;
+  g.rule = P.S.rule;
+  g.rule_total = P.S.rule_total;
+  for (int m = 0; m < 4; ++m) {
+    g.rule_off[m] = P.S.rule_off[m];
+    g.rule_n[m] = P.S.rule_n[m];
+  }
+  g.centred = P.centred;
+  g.Q = P.Q;
+  g.device = P.device;
+  g.stream = P.stream;
+  return g;
+}
+
 void assemble_device(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test, void* dev_out,
                      bool accumulate, void* stream) {
   if (!dev_out) throw Error("output pointer is null");
@@ -521,8 +701,13 @@ void assemble_device(const Assembler& a, const FunctionSpace& trial, const Funct
       const int nta = (R.n_test + dev::kTile - 1) / dev::kTile, ntb = (R.n_trial + dev::kTile - 1) / dev::kTile;
       int ntiles = 0;
       const int2* tiles = tile_list(P.device, nta, ntb, sym, &ntiles);
-      launch_regular(a.op, a.kind == kHelmholtz, P.centred, true, P.Q, dim3(ntiles), P.stream, R, tiles,
-                     sym ? 1 : 0);
+      if (P.mixed) {
+        launch_mixed(a.op, a.kind == kHelmholtz, P.centred, P.Q, dim3(ntiles), P.stream, R, tiles, sym ? 1 : 0,
+                     P.near, P.nnear);
+      } else {
+        launch_regular(a.op, a.kind == kHelmholtz, P.centred, true, P.Q, dim3(ntiles), P.stream, R, tiles,
+                       sym ? 1 : 0);
+      }
     } else {
       launch_regular(a.op, a.kind == kHelmholtz, P.centred, false, P.Q, grid, P.stream, R);
     }

@@ -36,6 +36,8 @@ enum Mode { kStore = 0, kAccumulate = 1, kAtomic = 2, kCoo = 3 };
 struct RegularArgs {
   const double* test_pts;   // [n_test][Q][4]: x, y, z, w_p * jdet
   const double* trial_pts;  // [n_trial][Q][4]
+  const float* test_pts32;  // [n_test][Q][4]: x - centroid (3), w_p * jdet, FP32 (mixed precision)
+  const float* trial_pts32; // [n_trial][Q][4]
   const double* test_geo;   // [n_test][kGeoStride]
   const double* trial_geo;  // [n_trial][kGeoStride]
   const int* test_vid;      // [n_test][4]
@@ -533,10 +535,29 @@ __global__ void __launch_bounds__(256) singular_kernel(const SingularArgs A) {
   }
 }
 
-__global__ void fill_zero_kernel(double* p, long long n) {
+// (a template so that every translation unit including this header may instantiate it)
+template <typename T = double>
+__global__ void fill_zero_kernel(T* p, long long n) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     p[i] = 0.0;
 }
 
 }  // namespace dev
+
+// Device inputs of the matrix-free kernels (apply, shape gradient), filled from the
+// assembly caches by gradient_inputs() in device.hip.
+struct GradientInputs {
+  dev::RegularArgs R;
+  const int4* pairs;
+  int npairs;
+  const double* rule;
+  int rule_total;
+  int rule_off[4];
+  int rule_n[4];
+  bool centred;
+  int Q;
+  int device;
+  void* stream;
+};
+
 }  // namespace bempp_amd

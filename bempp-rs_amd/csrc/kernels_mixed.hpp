@@ -1,0 +1,245 @@
+// kernels_mixed.hpp -- mixed-precision regular-pair assembly (P0 x P0).
+//
+// gfx950 issues FP32 FMAs and FP32 transcendentals at twice the FP64 rate, so far
+// cell pairs are evaluated in FP32 while everything that needs FP64 stays FP64:
+//   per cell pair (FP64): D = c_test - c_trial (centroids), R = |D|, kR, e^{ikR};
+//   per quadrature-point pair (FP32), with offsets o = x - c of each point from its
+//   cell centroid (stored in FP32) and e = o_p - o_q:
+//     s = e . (2D + e) = r^2 - R^2        (no cancellation: |e| << R)
+//     r^2 = R^2 + s, 1/r = rsq(r^2)
+//     delta = r - R = s / (r + R),  phase k delta (|k delta| <= 0.5: short series)
+//   the 6 trial points of a test point are summed in FP32, the rows in FP64, and the
+//   cell-pair sum is rotated by e^{ikR} in FP64.
+// Far means R >= kNearFactor (rad_test + rad_trial): then |e| <= R/3, r >= 2R/3 and the
+// FP32 terms carry relative errors of a few 1e-7. Nearer (and adjacent) pairs are
+// written as zero here and assembled in FP64 by near_pair_kernel (and the Duffy kernel).
+#pragma once
+#include "kernels.hpp"
+
+namespace bempp_amd {
+namespace dev {
+
+constexpr double kNearFactor = 3.0;
+
+// Far-pair predicate shared by the mixed tile kernel and the near-pair kernel.
+// Written with explicit fma so both evaluate it bitwise identically: every
+// non-adjacent pair is taken by exactly one of them.
+__device__ __forceinline__ bool far_pair(const double* ct, const double* cs, double trad, double srad,
+                                         double* R2_out) {
+  const double dx = ct[0] - cs[0], dy = ct[1] - cs[1], dz = ct[2] - cs[2];
+  const double R2 = fma(dx, dx, fma(dy, dy, dz * dz));
+  const double near = kNearFactor * (trad + srad);
+  *R2_out = R2;
+  return R2 >= near * near;
+}
+
+struct cplxf {
+  float re, im;
+};
+
+// sin/cos for |x| <= 0.5 in FP32: Taylor to x^7 / x^8 (truncation below 1e-8).
+__device__ __forceinline__ void sincos_small_f32(float x, float* s, float* c) {
+  const float x2 = x * x;
+  float ps = fmaf(x2, -1.9841270e-4f, 8.3333333e-3f);
+  ps = fmaf(ps, x2, -1.6666667e-1f);
+  *s = fmaf(ps * x2, x, x);
+  float pc = fmaf(x2, 2.4801587e-5f, -1.3888889e-3f);
+  pc = fmaf(pc, x2, 4.1666667e-2f);
+  pc = fmaf(pc, x2, -0.5f);
+  *c = fmaf(pc, x2, 1.0f);
+}
+
+// FP32 kernel scalar of one far quadrature-point pair, without e^{ikR} and 1/(4 pi).
+template <int OP, bool HELM>
+__device__ __forceinline__ cplxf kernel_scalar_f32(float ex, float ey, float ez, const float (&D)[3], float R2,
+                                                   float R, float k, const float* nx, const float* ny) {
+  const float fx = fmaf(2.0f, D[0], ex), fy = fmaf(2.0f, D[1], ey), fz = fmaf(2.0f, D[2], ez);
+  const float s = fmaf(ex, fx, fmaf(ey, fy, ez * fz));
+  const float r2 = R2 + s;
+  const float rinv = __builtin_amdgcn_rsqf(r2);
+  cplxf g;
+  float kr = 0.0f, sn = 0.0f, cs = 1.0f;
+  if constexpr (HELM) {
+    const float r = r2 * rinv;
+    const float delta = s * __builtin_amdgcn_rcpf(r + R);
+    sincos_small_f32(k * delta, &sn, &cs);
+    g = {cs * rinv, sn * rinv};
+    if constexpr (OP == 1 || OP == 2) kr = k * r;
+  } else {
+    g = {rinv, 0.0f};
+  }
+  if constexpr (OP == 0 || OP == 3) {
+    return g;
+  } else {
+    const float dx = D[0] + ex, dy = D[1] + ey, dz = D[2] + ez;
+    float dn;
+    if constexpr (OP == 1)
+      dn = fmaf(dx, ny[0], fmaf(dy, ny[1], dz * ny[2]));
+    else
+      dn = -fmaf(dx, nx[0], fmaf(dy, nx[1], dz * nx[2]));
+    const float f = dn * rinv * rinv * rinv;
+    if constexpr (HELM)
+      return {fmaf(kr, sn, cs) * f, fmaf(-kr, cs, sn) * f};
+    else
+      return {f, 0.0f};
+  }
+}
+
+// Mixed-precision P0 x P0 tiles: same tiling, symmetry and LDS mirror as
+// regular_p0_tiled; near and adjacent pairs are written as zero.
+template <int Q, int OP, bool HELM>
+__global__ void __launch_bounds__(256) regular_p0_tiled_mixed(const RegularArgs A, const int2* __restrict__ tiles,
+                                                              int symmetric) {
+  __shared__ double2 lds[4][kChunk][kTile];
+  const int2 tile = tiles[blockIdx.x];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i = tile.x * kTile + lane;
+  const bool i_ok = i < A.n_test;
+  const int il = i_ok ? i : A.n_test - 1;
+  float op[Q][3];
+  double wp[Q];
+#pragma unroll
+  for (int p = 0; p < Q; ++p) {
+    const float4 o = ((const float4*)A.test_pts32)[(size_t)il * Q + p];
+    op[p][0] = o.x;
+    op[p][1] = o.y;
+    op[p][2] = o.z;
+    wp[p] = A.test_pts[((size_t)il * Q + p) * 4 + 3];
+  }
+  const double* tgeo = A.test_geo + (size_t)il * kGeoStride;
+  const float nx[3] = {(float)tgeo[9], (float)tgeo[10], (float)tgeo[11]};
+  const double ct[3] = {tgeo[kGeoCentroid], tgeo[kGeoCentroid + 1], tgeo[kGeoCentroid + 2]};
+  const double trad = tgeo[kGeoRadius];
+  const int tv0 = A.test_vid[4 * il], tv1 = A.test_vid[4 * il + 1], tv2 = A.test_vid[4 * il + 2];
+  const bool mirror = symmetric && tile.x != tile.y;
+  // The FP32 trial data are read with ordinary (vector) loads so that they live in
+  // VGPRs: gfx950 dual-issues FP32 VALU operations only without SGPR operands.
+  const float4* trial_pts32 = (const float4*)A.trial_pts32;
+  const auto trial_geo = constant_view(A.trial_geo);
+  const auto trial_vid = constant_view(A.trial_vid);
+  float kf;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(kf) : "s"((float)A.k));
+  for (int c = 0; c < 2; ++c) {
+    const int jbase = tile.y * kTile + wave * 16 + c * kChunk;
+#pragma unroll 1
+    for (int jj = 0; jj < kChunk; ++jj) {
+      const int j = jbase + jj;
+      if (j >= A.n_trial) {
+        lds[wave][jj][lane] = make_double2(0.0, 0.0);
+        continue;
+      }
+      float sp[Q * 4];
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const float4 t = trial_pts32[(size_t)j * Q + q];
+        sp[4 * q] = t.x;
+        sp[4 * q + 1] = t.y;
+        sp[4 * q + 2] = t.z;
+        sp[4 * q + 3] = t.w;
+      }
+      const auto sg = trial_geo + (size_t)j * kGeoStride;
+      const double cs[3] = {sg[kGeoCentroid], sg[kGeoCentroid + 1], sg[kGeoCentroid + 2]};
+      const double srad = sg[kGeoRadius];
+      const float ny[3] = {(float)sg[9], (float)sg[10], (float)sg[11]};
+      const int sv0 = trial_vid[4 * j], sv1 = trial_vid[4 * j + 1], sv2 = trial_vid[4 * j + 2];
+      const bool adjacent = A.same_grid && (tv0 == sv0 || tv0 == sv1 || tv0 == sv2 || tv1 == sv0 || tv1 == sv1 ||
+                                            tv1 == sv2 || tv2 == sv0 || tv2 == sv1 || tv2 == sv2);
+      // cell-pair quantities in FP64
+      double R2;
+      const bool far = far_pair(ct, cs, trad, srad, &R2) && !adjacent;
+      const double Dx = ct[0] - cs[0], Dy = ct[1] - cs[1], Dz = ct[2] - cs[2];
+      cplx acc = {0.0, 0.0};
+      if (far) {
+        const double R = R2 * rsqrt_nr(R2);
+        const float D[3] = {(float)Dx, (float)Dy, (float)Dz};
+        const float R2f = (float)R2, Rf = (float)R;
+#pragma unroll
+        for (int p = 0; p < Q; ++p) {
+          cplxf ap = {0.0f, 0.0f};
+#pragma unroll
+          for (int q = 0; q < Q; ++q) {
+            const cplxf kv = kernel_scalar_f32<OP, HELM>(op[p][0] - sp[4 * q], op[p][1] - sp[4 * q + 1],
+                                                         op[p][2] - sp[4 * q + 2], D, R2f, Rf, kf, nx, ny);
+            ap.re = fmaf(kv.re, sp[4 * q + 3], ap.re);
+            ap.im = fmaf(kv.im, sp[4 * q + 3], ap.im);
+          }
+          acc.re = fma((double)ap.re, wp[p], acc.re);
+          acc.im = fma((double)ap.im, wp[p], acc.im);
+        }
+        if constexpr (HELM) {
+          cplx e;
+          sincos_cw(A.k * R, &e.im, &e.re);
+          acc = cmul(acc, e);
+        }
+        if constexpr (OP == 3) {
+          const double nn = HELM ? -A.k * A.k * (tgeo[9] * sg[9] + tgeo[10] * sg[10] + tgeo[11] * sg[11]) : 0.0;
+          acc = {nn * acc.re, nn * acc.im};
+        }
+      }
+      const cplx v = {acc.re * kInv4Pi, acc.im * kInv4Pi};
+      if (i_ok) store_value(A.out, (long long)i + A.ld * (long long)j, v, A.out_type, A.mode);
+      if (mirror) lds[wave][jj][lane] = make_double2(v.re, v.im);
+    }
+    if (mirror) {
+      __syncthreads();
+      const int row = lane & (kChunk - 1);
+      const int jr = jbase + row;
+#pragma unroll
+      for (int s = 0; s < kTile / kChunk; ++s) {
+        const int col = s * kChunk + (lane >> 3);
+        const int ic = tile.x * kTile + col;
+        const double2 val = lds[wave][row][col];
+        if (jr < A.n_trial && ic < A.n_test)
+          store_value(A.out, (long long)jr + A.ld * (long long)ic, {val.x, val.y}, A.out_type, A.mode);
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// FP64 evaluation of a list of near (non-adjacent) P0 x P0 cell pairs, one
+// thread per pair, added to their (unique) entries.
+template <int Q, int OP, bool HELM, bool CENTRED>
+__global__ void __launch_bounds__(256) near_pair_kernel(const RegularArgs A, const int2* __restrict__ pairs,
+                                                        int npairs) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npairs) return;
+  const int2 pr = pairs[t];
+  const int i = pr.x, j = pr.y;
+  const double* tg = A.test_geo + (size_t)i * kGeoStride;
+  const double* sg = A.trial_geo + (size_t)j * kGeoStride;
+  double R2;
+  // the host list is a superset; keep exactly the pairs the tile kernel left out
+  if (far_pair(tg + kGeoCentroid, sg + kGeoCentroid, tg[kGeoRadius], sg[kGeoRadius], &R2)) return;
+  double xp[Q][3], wp[Q], sp[Q * 4];
+#pragma unroll
+  for (int p = 0; p < Q; ++p) {
+    const double* s = A.test_pts + ((size_t)i * Q + p) * 4;
+    xp[p][0] = s[0];
+    xp[p][1] = s[1];
+    xp[p][2] = s[2];
+    wp[p] = s[3];
+  }
+#pragma unroll
+  for (int m = 0; m < Q * 4; ++m) sp[m] = A.trial_pts[(size_t)j * Q * 4 + m];
+  const double nx[3] = {tg[9], tg[10], tg[11]};
+  const double ny[3] = {sg[9], sg[10], sg[11]};
+  cplx acc;
+  if constexpr (CENTRED) {
+    double kR;
+    const cplx e = pair_phase(tg + kGeoCentroid, sg + kGeoCentroid, A.k, &kR);
+    acc = cmul(p0_pair<Q, OP, HELM, true>(xp, wp, nx, sp, ny, A.k, kR), e);
+  } else {
+    acc = p0_pair<Q, OP, HELM, false>(xp, wp, nx, sp, ny, A.k, 0.0);
+  }
+  if constexpr (OP == 3) {
+    const double nn = HELM ? -A.k * A.k * (nx[0] * ny[0] + nx[1] * ny[1] + nx[2] * ny[2]) : 0.0;
+    acc = {nn * acc.re, nn * acc.im};
+  }
+  store_value(A.out, (long long)i + A.ld * (long long)j, {acc.re * kInv4Pi, acc.im * kInv4Pi}, A.out_type,
+              kAccumulate);
+}
+
+}  // namespace dev
+}  // namespace bempp_amd

@@ -177,9 +177,17 @@ MatrixT* boundary_assembler_assemble(const BoundaryAssemblerT* assembler, const 
 MatrixT* boundary_assembler_assemble_singular(const BoundaryAssemblerT* assembler, const SpaceT* trial_space,
                                               const SpaceT* test_space);
 
+/* Matrix-free operator application y = A u (no counterpart in the reference; the
+ * FMM prototype of tests/fmm.rs.bak applies operators this way): A is recomputed on
+ * the device and never stored. P0 spaces; u has n_trial and y n_test entries of the
+ * assembler's dtype; y is overwritten. transpose != 0 applies A^T (single-layer and
+ * hypersingular operators on one space). */
+int boundary_assembler_apply(const BoundaryAssemblerT* assembler, const SpaceT* trial_space,
+                             const SpaceT* test_space, const void* u, void* y, int transpose);
+
 /* Shape gradient (no counterpart in the reference): grad[v][d] = d(v^T A(X) u)/dX[v][d]
- * for the operator of `assembler` on one grid (test and trial space on the same
- * grid), computed matrix-free on the device with recomputation. u has n_trial,
+ * for the single-layer operator of `assembler` on P0 spaces of one grid, computed
+ * matrix-free on the device with recomputation. u has n_trial,
  * v n_test entries of the assembler's dtype; grad is real f64 [npoints][3] for
  * Laplace and complex128 [npoints][3] for Helmholtz (d/dX of the complex bilinear form). */
 int boundary_assembler_shape_gradient(const BoundaryAssemblerT* assembler, const SpaceT* trial_space,

@@ -191,3 +191,44 @@ def test_deterministic_p0():
     s = _space(g, "dp0")
     a = _assembler("helmholtz", "double_layer", 5.0, _options())
     assert np.array_equal(a.assemble(s, s), a.assemble(s, s))
+
+
+def _mixed_options(nq=6, ns=4):
+    from bempp_amd.assembly.boundary import Precision
+
+    o = _options(nq, ns)
+    o.set_precision(Precision.Mixed)
+    return o
+
+
+@pytest.mark.parametrize("kind,k", [("laplace", 0.0), ("helmholtz", 2.5)])
+@pytest.mark.parametrize("op", list(OPS))
+def test_mixed_precision_p0(kind, k, op):
+    """Mixed precision (FP32 far pairs, FP64 near and adjacent pairs) on the level-4 icosphere
+    (5120 cells; k = 2.5 satisfies the centred-phase bound): relative Frobenius error <= 1e-6
+    against the FP64 oracle, and no entry dropped or counted twice (largest entrywise error
+    relative to the largest entry <= 1e-5)."""
+    g = icosphere(4)
+    s = _space(g, "dp0")
+    a = _assembler(kind, op, k, _mixed_options())
+    print(f"[mixed] {kind} {op} k={k}", flush=True)
+    A = a.assemble(s, s)
+    R = O.assemble(kind, op, g.points, g.cells, k=k)
+    if np.linalg.norm(R) == 0.0:
+        assert np.abs(A).max() == 0.0
+        return
+    err = _rel(A, R)
+    emax = np.abs(A - R).max() / np.abs(R).max()
+    print(f"[mixed]   rel Frobenius err {err:.3e}, max entry err {emax:.3e}", flush=True)
+    assert err <= 1e-6 and emax <= 1e-5
+    assert err > 1e-12  # the FP32 path ran
+
+
+def test_mixed_falls_back_to_fp64():
+    """Coarse mesh with k = 5 (|k| (rad_i + rad_j) > 0.5): the mixed option falls back to FP64."""
+    g = icosphere(2)
+    s = _space(g, "dp0")
+    a = _assembler("helmholtz", "single_layer", 5.0, _mixed_options())
+    A = a.assemble(s, s)
+    R = O.assemble("helmholtz", "single_layer", g.points, g.cells, k=5.0)
+    assert _rel(A, R) < TOL
