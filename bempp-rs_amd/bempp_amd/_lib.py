@@ -73,6 +73,16 @@ _SIGNATURES = {
     "boundary_assembler_apply": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_int]),
     "boundary_assembler_set_profiling": (c_int, [c_vp, c_int]),
     "boundary_assembler_kernel_times": (c_int, [c_vp, c_vp]),
+    "boundary_assembler_batch_new": (c_vp, [c_vp, c_vp, c_vp, c_size]),
+    "boundary_assembler_batch_t_free": (None, [c_vp]),
+    "boundary_assembler_batch_assemble": (c_int, [c_vp, c_vp]),
+    "boundary_assembler_batch_set_graph": (c_int, [c_vp, c_int]),
+    "boundary_assembler_batch_set_launch": (c_int, [c_vp, c_int, c_int, c_int]),
+    "boundary_assembler_batch_get_launch": (c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "boundary_assembler_batch_counts": (c_int, [c_vp, c_vp]),
+    "boundary_assembler_batch_autotune": (ctypes.c_double, [c_vp, ctypes.c_char_p, c_int, c_vp]),
+    "boundary_assembler_batch_set_profiling": (c_int, [c_vp, c_int]),
+    "boundary_assembler_batch_kernel_times": (c_int, [c_vp, c_vp]),
     "bempp_triangle_rule": (c_int, [c_int, c_vp, c_vp]),
     "bempp_canonical_pair_rule": (c_int, [c_int, c_int, c_vp]),
 }

@@ -389,6 +389,130 @@ int boundary_assembler_kernel_times(const BoundaryAssemblerT* a, float* ms) {
       1);
 }
 
+// ---- batched assembly ----
+struct BoundaryAssemblerBatchT {
+  Batch* b;
+};
+
+BoundaryAssemblerBatchT* boundary_assembler_batch_new(const BoundaryAssemblerT* const* assemblers,
+                                                      const SpaceT* const* spaces, void* const* outputs, size_t n) {
+  return guarded(
+      [&]() -> BoundaryAssemblerBatchT* {
+        need(assemblers, "assemblers");
+        need(spaces, "spaces");
+        need(outputs, "outputs");
+        std::vector<const Assembler*> a(n);
+        std::vector<const FunctionSpace*> s(n);
+        std::vector<void*> o(outputs, outputs + n);
+        for (size_t i = 0; i < n; ++i) {
+          need(assemblers[i], "assembler");
+          need(spaces[i], "space");
+          a[i] = assemblers[i]->a;
+          s[i] = spaces[i]->s;
+        }
+        Batch* b = batch_new(a, s, o);
+        return new BoundaryAssemblerBatchT{b};
+      },
+      nullptr);
+}
+
+void boundary_assembler_batch_t_free(BoundaryAssemblerBatchT* batch) {
+  if (!batch) return;
+  batch_free(batch->b);
+  delete batch;
+}
+
+int boundary_assembler_batch_assemble(BoundaryAssemblerBatchT* batch, void* stream) {
+  return guarded(
+      [&]() {
+        need(batch, "batch");
+        batch_assemble(*batch->b, stream);
+        return 0;
+      },
+      1);
+}
+
+int boundary_assembler_batch_set_graph(BoundaryAssemblerBatchT* batch, int enable) {
+  return guarded(
+      [&]() {
+        need(batch, "batch");
+        batch_set_graph(*batch->b, enable != 0);
+        return 0;
+      },
+      1);
+}
+
+int boundary_assembler_batch_set_launch(BoundaryAssemblerBatchT* batch, int blocks_per_cu, int chunk, int order) {
+  return guarded(
+      [&]() {
+        need(batch, "batch");
+        batch_set_launch(*batch->b, blocks_per_cu, chunk, order);
+        return 0;
+      },
+      1);
+}
+
+int boundary_assembler_batch_get_launch(const BoundaryAssemblerBatchT* batch, int* blocks_per_cu, int* chunk,
+                                        int* order) {
+  return guarded(
+      [&]() {
+        need(batch, "batch");
+        need(blocks_per_cu, "blocks_per_cu");
+        need(chunk, "chunk");
+        need(order, "order");
+        batch_get_launch(*batch->b, blocks_per_cu, chunk, order);
+        return 0;
+      },
+      1);
+}
+
+int boundary_assembler_batch_counts(const BoundaryAssemblerBatchT* batch, int64_t* counts) {
+  return guarded(
+      [&]() {
+        need(batch, "batch");
+        need(counts, "counts");
+        long long c[4];
+        batch_counts(*batch->b, c);
+        for (int i = 0; i < 4; ++i) counts[i] = (int64_t)c[i];
+        return 0;
+      },
+      1);
+}
+
+double boundary_assembler_batch_autotune(BoundaryAssemblerBatchT* batch, const char* cache_path, int reps,
+                                         int* from_cache) {
+  return guarded(
+      [&]() {
+        need(batch, "batch");
+        int fc = 0;
+        const double ms = batch_autotune(*batch->b, cache_path, reps, &fc);
+        if (from_cache) *from_cache = fc;
+        return ms;
+      },
+      -1.0);
+}
+
+int boundary_assembler_batch_set_profiling(BoundaryAssemblerBatchT* batch, int enable) {
+  return guarded(
+      [&]() {
+        need(batch, "batch");
+        batch_set_profiling(*batch->b, enable != 0);
+        return 0;
+      },
+      1);
+}
+
+int boundary_assembler_batch_kernel_times(const BoundaryAssemblerBatchT* batch, float* ms) {
+  return guarded(
+      [&]() {
+        need(batch, "batch");
+        need(ms, "ms");
+        batch_kernel_times(*batch->b, ms);
+        return 0;
+      },
+      1);
+}
+
 // ---- matrices ----
 size_t matrix_shape0(const MatrixT* m) { return m ? m->shape0 : 0; }
 size_t matrix_shape1(const MatrixT* m) { return m ? m->shape1 : 0; }

@@ -159,6 +159,21 @@ int device_count();
 // profiled assembly (waits for them); -1 when not recorded.
 void kernel_times(const Assembler& a, float* ms2);
 
+// Batched dense assembly (batch.hip): items (assembler, P0 space, device output) share
+// kernel, operator, dtype, precision and regular rule; each output is overwritten.
+struct Batch;
+Batch* batch_new(const std::vector<const Assembler*>& asms, const std::vector<const FunctionSpace*>& spaces,
+                 const std::vector<void*>& outputs);
+void batch_free(Batch* b);
+void batch_assemble(Batch& b, void* stream);
+void batch_set_graph(Batch& b, bool enable);
+void batch_set_profiling(Batch& b, bool enable);
+void batch_set_launch(Batch& b, int blocks_per_cu, int chunk, int order);
+void batch_get_launch(const Batch& b, int* blocks_per_cu, int* chunk, int* order);
+void batch_counts(const Batch& b, long long* out4);
+void batch_kernel_times(const Batch& b, float* ms2);
+double batch_autotune(Batch& b, const char* cache_path, int reps, int* from_cache);
+
 inline bool is_complex(DType d) { return d == kC32 || d == kC64; }
 inline size_t dtype_size(DType d) { return d == kF32 ? 4 : (d == kF64 || d == kC32) ? 8 : 16; }
 

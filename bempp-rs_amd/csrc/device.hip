@@ -454,7 +454,8 @@ void launch_mixed_q(int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& A, 
                     const int2* near, int nnear) {
   dim3 ng((unsigned)((nnear + 255) / 256));
 #define BEMPP_LAUNCH(QQ)                                                                      \
-  dev::regular_p0_tiled_mixed<QQ, OP, HELM><<<grid, 256, 0, s>>>(A, tiles, sym);              \
+  if (CENTRED || !HELM) dev::regular_p0_tiled_mixed<QQ, OP, HELM, false><<<grid, 256, 0, s>>>(A, tiles, sym); \
+  else dev::regular_p0_tiled_mixed<QQ, OP, HELM, true><<<grid, 256, 0, s>>>(A, tiles, sym);            \
   if (nnear > 0) dev::near_pair_kernel<QQ, OP, HELM, CENTRED><<<ng, 256, 0, s>>>(A, near, nnear);
   switch (Q) {
     case 1: BEMPP_LAUNCH(1); break;
@@ -482,7 +483,7 @@ void launch_mixed(int op, bool helm, bool centred, int Q, dim3 grid, hipStream_t
                   const int2* tiles, int sym, const int2* near, int nnear) {
   if (!helm) launch_mixed_op<false, false>(op, Q, grid, s, A, tiles, sym, near, nnear);
   else if (centred) launch_mixed_op<true, true>(op, Q, grid, s, A, tiles, sym, near, nnear);
-  else throw Error("mixed precision needs the centred-phase bound |k| (rad_i + rad_j) <= 0.5");
+  else launch_mixed_op<true, false>(op, Q, grid, s, A, tiles, sym, near, nnear);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -512,20 +513,7 @@ void launch_singular(int op, bool helm, bool p0p0, bool small, dim3 grid, hipStr
   HIP_CHECK(hipGetLastError());
 }
 
-struct Prepared {
-  int device;
-  bool centred;         // Helmholtz phases taken relative to the cell-pair centroid distance
-  bool small_adjacent;  // |k| |x - y| <= 0.5 on every adjacent cell pair
-  bool mixed;           // FP32 far pairs + FP64 near-pair list (P0 x P0 on one grid)
-  const int2* near;
-  int nnear;
-  hipStream_t stream;
-  int Q;
-  bool same_grid;
-  dev::RegularArgs R;
-  dev::SingularArgs S;
-  int npairs;
-};
+}  // namespace
 
 void check_assembler(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test) {
   if (a.op != kSL && a.op != kDL && a.op != kADL && a.op != kHS)
@@ -576,9 +564,8 @@ Prepared prepare(const Assembler& a, const FunctionSpace& trial_c, const Functio
   dev::RegularArgs& R = P.R;
   R.test_pts = grid_points_for_rule(tgrid, tc, P.Q, xi, w);
   R.trial_pts = grid_points_for_rule(sgrid, sc, P.Q, xi, w);
-  // mixed precision: P0 x P0 on one grid, and for Helmholtz the centred phase bound
-  P.mixed = a.options.precision == kMixed && test.nb == 1 && trial.nb == 1 && P.same_grid &&
-            (a.kind == kLaplace || P.centred);
+  // mixed precision: P0 x P0 on one grid
+  P.mixed = a.options.precision == kMixed && test.nb == 1 && trial.nb == 1 && P.same_grid;
   P.near = nullptr;
   P.nnear = 0;
   R.test_pts32 = R.trial_pts32 = nullptr;
@@ -635,9 +622,6 @@ Prepared prepare(const Assembler& a, const FunctionSpace& trial_c, const Functio
   return P;
 }
 
-}  // namespace
-
-
 // Geometry, rule tables and adjacent pairs for the matrix-free kernels (gradient.hip).
 GradientInputs gradient_inputs(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test) {
   Prepared P = prepare(a, trial, test, nullptr, nullptr);
@@ -662,6 +646,7 @@ void assemble_device(const Assembler& a, const FunctionSpace& trial, const Funct
                      bool accumulate, void* stream) {
   if (!dev_out) throw Error("output pointer is null");
   Prepared P = prepare(a, trial, test, dev_out, stream);
+  P.stream = (hipStream_t)stream;  // device entry point: the caller's stream, NULL = the null stream
   const bool p0p0 = test.nb == 1 && trial.nb == 1;
   const long long n_out = (long long)test.size * (long long)trial.size * (is_complex(a.dtype) ? 2 : 1);
   if (!p0p0 && !accumulate) {

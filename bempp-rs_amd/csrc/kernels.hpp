@@ -196,6 +196,21 @@ __device__ __forceinline__ const __attribute__((address_space(4))) T* constant_v
   return (const __attribute__((address_space(4))) T*)p;
 }
 
+// Wave-uniform copy of a small struct from read-only device memory with scalar loads
+// (a struct cannot be copied from the constant address space directly).
+template <typename T>
+__device__ __forceinline__ T load_uniform(const T* p) {
+  static_assert(sizeof(T) % 4 == 0, "struct size must be a multiple of 4 bytes");
+  constexpr int kWords = sizeof(T) / 4;
+  const auto w = constant_view((const int*)p);
+  int words[kWords];
+#pragma unroll
+  for (int i = 0; i < kWords; ++i) words[i] = w[i];
+  T v;
+  __builtin_memcpy(&v, words, sizeof(T));
+  return v;
+}
+
 // e^{ikR} for the centroid distance R of a cell pair (CENTRED Helmholtz pairs).
 __device__ __forceinline__ cplx pair_phase(const double* ct, const double* cs, double k, double* kR) {
   const double dx = ct[0] - cs[0], dy = ct[1] - cs[1], dz = ct[2] - cs[2];
@@ -339,11 +354,13 @@ __device__ __forceinline__ cplx p0_pair(const double (&xp)[Q][3], const double (
   return acc;
 }
 
-template <int Q, int OP, bool HELM, bool CENTRED>
-__global__ void __launch_bounds__(256) regular_p0_tiled(const RegularArgs A, const int2* __restrict__ tiles,
-                                                        int symmetric) {
-  __shared__ double2 lds[4][kChunk][kTile];
-  const int2 tile = tiles[blockIdx.x];
+// CH: trial cells per chunk (a wavefront covers 16 trial cells in 16 / CH chunks);
+// the mirror transpose stores CH contiguous entries per segment.
+// Args: RegularArgs, possibly qualified with the constant address space (batched
+// launches read each item's arguments in place, with scalar loads).
+template <int Q, int OP, bool HELM, bool CENTRED, int CH = kChunk, typename Args>
+__device__ __forceinline__ void p0_tile(Args& A, const int2 tile, int symmetric,
+                                        double2 (&lds)[4][CH][kTile]) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: lets trial loads be scalar
   const int i = tile.x * kTile + lane;
@@ -366,10 +383,11 @@ __global__ void __launch_bounds__(256) regular_p0_tiled(const RegularArgs A, con
   const auto trial_pts = constant_view(A.trial_pts);
   const auto trial_geo = constant_view(A.trial_geo);
   const auto trial_vid = constant_view(A.trial_vid);
-  for (int c = 0; c < 2; ++c) {
-    const int jbase = tile.y * kTile + wave * 16 + c * kChunk;
+  static_assert(CH == 4 || CH == 8 || CH == 16, "chunk must divide 16 and 64");
+  for (int c = 0; c < 16 / CH; ++c) {
+    const int jbase = tile.y * kTile + wave * 16 + c * CH;
 #pragma unroll 1
-    for (int jj = 0; jj < kChunk; ++jj) {
+    for (int jj = 0; jj < CH; ++jj) {
       const int j = jbase + jj;
       if (j >= A.n_trial) {  // wave-uniform
         lds[wave][jj][lane] = make_double2(0.0, 0.0);
@@ -407,11 +425,11 @@ __global__ void __launch_bounds__(256) regular_p0_tiled(const RegularArgs A, con
     }
     if (mirror) {  // block-uniform
       __syncthreads();
-      const int row = lane & (kChunk - 1);
+      const int row = lane % CH;
       const int jr = jbase + row;  // row of the mirrored entry (a trial cell of this tile)
 #pragma unroll
-      for (int s = 0; s < kTile / kChunk; ++s) {
-        const int col = s * kChunk + (lane >> 3);
+      for (int s = 0; s < CH; ++s) {
+        const int col = s * (kTile / CH) + lane / CH;
         const int ic = tile.x * kTile + col;  // column of the mirrored entry (a test cell)
         const double2 val = lds[wave][row][col];
         if (jr < A.n_trial && ic < A.n_test)
@@ -420,6 +438,48 @@ __global__ void __launch_bounds__(256) regular_p0_tiled(const RegularArgs A, con
       __syncthreads();
     }
   }
+}
+
+template <int Q, int OP, bool HELM, bool CENTRED>
+__global__ void __launch_bounds__(256) regular_p0_tiled(const RegularArgs A, const int2* __restrict__ tiles,
+                                                        int symmetric) {
+  __shared__ double2 lds[4][kChunk][kTile];
+  p0_tile<Q, OP, HELM, CENTRED>(A, tiles[blockIdx.x], symmetric, lds);
+}
+
+// ---------------------------------------------------------------------------
+// Batched P0 x P0 assembly: many independent meshes/wavenumbers in one persistent
+// launch. Tiles of all items form one list {item, tile row, tile column, _};
+// workgroups take tiles from a global counter until the list is exhausted (every
+// workgroup exits once the counter passes the end, so the grid always drains).
+// The per-item arguments live in device memory and are read with scalar loads.
+// Items are grouped by phase mode: one launch per CENTRED value, so that each
+// kernel carries the registers of one tile variant only.
+// ---------------------------------------------------------------------------
+struct BatchTile {
+  int item, a, b, pad;
+};
+
+template <int Q, int OP, bool HELM, bool CENTRED, int CH>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) batched_p0_persistent(const RegularArgs* __restrict__ items,
+                                                             const BatchTile* __restrict__ tiles, int ntiles,
+                                                             int* counter, int symmetric) {
+  __shared__ double2 lds[4][CH][kTile];
+  __shared__ int next;
+  for (;;) {
+    if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+    __syncthreads();
+    const int t = __builtin_amdgcn_readfirstlane(next);
+    __syncthreads();  // everyone has read `next` before thread 0 overwrites it
+    if (t >= ntiles) return;
+    const BatchTile bt = load_uniform(tiles + t);
+    p0_tile<Q, OP, HELM, CENTRED, CH>(constant_view(items)[bt.item], make_int2(bt.a, bt.b), symmetric, lds);
+  }
+}
+
+template <typename T = int>
+__global__ void reset_counter_kernel(T* counter, int n) {
+  if (blockIdx.x == 0 && (int)threadIdx.x < n) counter[threadIdx.x] = 0;
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -439,11 +499,8 @@ __device__ __forceinline__ double wave_sum(double v) {
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int perm_digit(int code, int k) { return (code >> (2 * k)) & 3; }
 
-template <int OP, bool HELM, bool P0P0, bool SMALL>
-__global__ void __launch_bounds__(256) singular_kernel(const SingularArgs A) {
-  const int pair = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6));
-  const int lane = threadIdx.x & 63;
-  if (pair >= A.npairs) return;  // uniform per wavefront
+template <int OP, bool HELM, bool P0P0, bool SMALL, typename Args>
+__device__ __forceinline__ void singular_pair(Args& A, const int pair, const int lane) {
   const auto pw = constant_view((const int*)A.pairs) + 4 * (size_t)pair;
   const int tc = pw[0], sc = pw[1], m = pw[2], code = pw[3];
   const auto tg = constant_view(A.test_geo) + (size_t)tc * kGeoStride;
@@ -461,7 +518,10 @@ __global__ void __launch_bounds__(256) singular_kernel(const SingularArgs A) {
   const double nx[3] = {tg[9], tg[10], tg[11]};
   const double ny[3] = {sg[9], sg[10], sg[11]};
   const double jj = tg[12] * sg[12];
-  const int off = A.rule_off[m], n = A.rule_n[m], tot = A.rule_total;
+  // selects rather than a dynamic index, so a struct copy of A stays in registers
+  const int off = m == 1 ? A.rule_off[1] : (m == 2 ? A.rule_off[2] : A.rule_off[3]);
+  const int n = m == 1 ? A.rule_n[1] : (m == 2 ? A.rule_n[2] : A.rule_n[3]);
+  const int tot = A.rule_total;
   constexpr int NB = P0P0 ? 1 : 3;
   cplx M[NB][NB];
 #pragma unroll
@@ -535,6 +595,36 @@ __global__ void __launch_bounds__(256) singular_kernel(const SingularArgs A) {
   }
 }
 
+template <int OP, bool HELM, bool P0P0, bool SMALL>
+__global__ void __launch_bounds__(256) singular_kernel(const SingularArgs A) {
+  const int pair = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6));
+  if (pair >= A.npairs) return;  // uniform per wavefront
+  singular_pair<OP, HELM, P0P0, SMALL>(A, pair, threadIdx.x & 63);
+}
+
+// Adjacent P0 x P0 pairs of a batch: one wavefront per pair of the concatenated
+// pair lists (item i owns pairs [first[i], first[i + 1])); SMALL is chosen per item.
+template <int OP, bool HELM>
+__global__ void __launch_bounds__(256) batched_singular_kernel(const SingularArgs* __restrict__ items,
+                                                               const int* __restrict__ small,
+                                                               const int* __restrict__ first, int nitems) {
+  const int g = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6));
+  const auto fv = constant_view(first);
+  if (g >= fv[nitems]) return;  // uniform per wavefront
+  int lo = 0, hi = nitems - 1;  // last item with first[item] <= g
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (fv[mid] <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  const auto& A = constant_view(items)[lo];
+  const int lane = threadIdx.x & 63;
+  if (HELM && constant_view(small)[lo])
+    singular_pair<OP, HELM, true, true>(A, g - fv[lo], lane);
+  else
+    singular_pair<OP, HELM, true, false>(A, g - fv[lo], lane);
+}
+
 // (a template so that every translation unit including this header may instantiate it)
 template <typename T = double>
 __global__ void fill_zero_kernel(T* p, long long n) {
@@ -559,5 +649,28 @@ struct GradientInputs {
   int device;
   void* stream;
 };
+
+struct Assembler;      // core.hpp
+struct FunctionSpace;  // core.hpp
+
+// Device inputs of one dense assembly, filled by prepare() in device.hip from the
+// grid/space caches (geometry, points, DOFs, adjacent pairs, rule tables).
+struct Prepared {
+  int device;
+  bool centred;         // Helmholtz phases taken relative to the cell-pair centroid distance
+  bool small_adjacent;  // |k| |x - y| <= 0.5 on every adjacent cell pair
+  bool mixed;           // FP32 far pairs + FP64 near-pair list (P0 x P0 on one grid)
+  const int2* near;
+  int nnear;
+  hipStream_t stream;
+  int Q;
+  bool same_grid;
+  dev::RegularArgs R;
+  dev::SingularArgs S;
+  int npairs;
+};
+
+Prepared prepare(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test, void* out,
+                 void* stream);
 
 }  // namespace bempp_amd

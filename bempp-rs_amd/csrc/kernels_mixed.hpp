@@ -7,7 +7,8 @@
 //   cell centroid (stored in FP32) and e = o_p - o_q:
 //     s = e . (2D + e) = r^2 - R^2        (no cancellation: |e| << R)
 //     r^2 = R^2 + s, 1/r = rsq(r^2)
-//     delta = r - R = s / (r + R),  phase k delta (|k delta| <= 0.5: short series)
+//     delta = r - R = s / (r + R),  phase k delta (|k delta| <= 0.5: short series;
+//     WIDE: any |k delta|, reduced by multiples of pi/2 in FP32 first)
 //   the 6 trial points of a test point are summed in FP32, the rows in FP64, and the
 //   cell-pair sum is rotated by e^{ikR} in FP64.
 // Far means R >= kNearFactor (rad_test + rad_trial): then |e| <= R/3, r >= 2R/3 and the
@@ -49,8 +50,31 @@ __device__ __forceinline__ void sincos_small_f32(float x, float* s, float* c) {
   *c = fmaf(pc, x2, 1.0f);
 }
 
+// sin/cos in FP32 for moderate |x| (the far-pair phase k delta, |k delta| <= |k| (rad_i + rad_j)):
+// x = n pi/2 + t with pi/2 split in two FP32 parts (each n C fused exactly), |t| <= pi/4,
+// Taylor series to t^9 / t^10 (truncation below 2e-9), quadrant by n mod 4.
+__device__ __forceinline__ void sincos_f32_wide(float x, float* s, float* c) {
+  const float n = __builtin_rintf(x * 0.63661977f);
+  float t = fmaf(-n, 1.5707964f, x);
+  t = fmaf(-n, -4.3711388e-8f, t);
+  const float t2 = t * t;
+  float ps = fmaf(t2, 2.7557319e-6f, -1.9841270e-4f);
+  ps = fmaf(ps, t2, 8.3333333e-3f);
+  ps = fmaf(ps, t2, -1.6666667e-1f);
+  const float st = fmaf(ps * t2, t, t);
+  float pc = fmaf(t2, -2.7557319e-7f, 2.4801587e-5f);
+  pc = fmaf(pc, t2, -1.3888889e-3f);
+  pc = fmaf(pc, t2, 4.1666667e-2f);
+  pc = fmaf(pc, t2, -0.5f);
+  const float ct = fmaf(pc, t2, 1.0f);
+  const int q = (int)n & 3;
+  const float sv = (q & 1) ? ct : st, cv = (q & 1) ? st : ct;
+  *s = (q & 2) ? -sv : sv;
+  *c = ((q + 1) & 2) ? -cv : cv;
+}
+
 // FP32 kernel scalar of one far quadrature-point pair, without e^{ikR} and 1/(4 pi).
-template <int OP, bool HELM>
+template <int OP, bool HELM, bool WIDE>
 __device__ __forceinline__ cplxf kernel_scalar_f32(float ex, float ey, float ez, const float (&D)[3], float R2,
                                                    float R, float k, const float* nx, const float* ny) {
   const float fx = fmaf(2.0f, D[0], ex), fy = fmaf(2.0f, D[1], ey), fz = fmaf(2.0f, D[2], ez);
@@ -62,7 +86,10 @@ __device__ __forceinline__ cplxf kernel_scalar_f32(float ex, float ey, float ez,
   if constexpr (HELM) {
     const float r = r2 * rinv;
     const float delta = s * __builtin_amdgcn_rcpf(r + R);
-    sincos_small_f32(k * delta, &sn, &cs);
+    if constexpr (WIDE)
+      sincos_f32_wide(k * delta, &sn, &cs);
+    else
+      sincos_small_f32(k * delta, &sn, &cs);
     g = {cs * rinv, sn * rinv};
     if constexpr (OP == 1 || OP == 2) kr = k * r;
   } else {
@@ -86,12 +113,10 @@ __device__ __forceinline__ cplxf kernel_scalar_f32(float ex, float ey, float ez,
 }
 
 // Mixed-precision P0 x P0 tiles: same tiling, symmetry and LDS mirror as
-// regular_p0_tiled; near and adjacent pairs are written as zero.
-template <int Q, int OP, bool HELM>
-__global__ void __launch_bounds__(256) regular_p0_tiled_mixed(const RegularArgs A, const int2* __restrict__ tiles,
-                                                              int symmetric) {
-  __shared__ double2 lds[4][kChunk][kTile];
-  const int2 tile = tiles[blockIdx.x];
+// p0_tile; near and adjacent pairs are written as zero.
+template <int Q, int OP, bool HELM, bool WIDE, int CH = kChunk, typename Args>
+__device__ __forceinline__ void p0_tile_mixed(Args& A, const int2 tile, int symmetric,
+                                              double2 (&lds)[4][CH][kTile]) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int i = tile.x * kTile + lane;
@@ -120,10 +145,10 @@ __global__ void __launch_bounds__(256) regular_p0_tiled_mixed(const RegularArgs 
   const auto trial_vid = constant_view(A.trial_vid);
   float kf;
   asm volatile("v_mov_b32 %0, %1" : "=v"(kf) : "s"((float)A.k));
-  for (int c = 0; c < 2; ++c) {
-    const int jbase = tile.y * kTile + wave * 16 + c * kChunk;
+  for (int c = 0; c < 16 / CH; ++c) {
+    const int jbase = tile.y * kTile + wave * 16 + c * CH;
 #pragma unroll 1
-    for (int jj = 0; jj < kChunk; ++jj) {
+    for (int jj = 0; jj < CH; ++jj) {
       const int j = jbase + jj;
       if (j >= A.n_trial) {
         lds[wave][jj][lane] = make_double2(0.0, 0.0);
@@ -159,7 +184,7 @@ __global__ void __launch_bounds__(256) regular_p0_tiled_mixed(const RegularArgs 
           cplxf ap = {0.0f, 0.0f};
 #pragma unroll
           for (int q = 0; q < Q; ++q) {
-            const cplxf kv = kernel_scalar_f32<OP, HELM>(op[p][0] - sp[4 * q], op[p][1] - sp[4 * q + 1],
+            const cplxf kv = kernel_scalar_f32<OP, HELM, WIDE>(op[p][0] - sp[4 * q], op[p][1] - sp[4 * q + 1],
                                                          op[p][2] - sp[4 * q + 2], D, R2f, Rf, kf, nx, ny);
             ap.re = fmaf(kv.re, sp[4 * q + 3], ap.re);
             ap.im = fmaf(kv.im, sp[4 * q + 3], ap.im);
@@ -183,11 +208,11 @@ __global__ void __launch_bounds__(256) regular_p0_tiled_mixed(const RegularArgs 
     }
     if (mirror) {
       __syncthreads();
-      const int row = lane & (kChunk - 1);
+      const int row = lane % CH;
       const int jr = jbase + row;
 #pragma unroll
-      for (int s = 0; s < kTile / kChunk; ++s) {
-        const int col = s * kChunk + (lane >> 3);
+      for (int s = 0; s < CH; ++s) {
+        const int col = s * (kTile / CH) + lane / CH;
         const int ic = tile.x * kTile + col;
         const double2 val = lds[wave][row][col];
         if (jr < A.n_trial && ic < A.n_test)
@@ -198,14 +223,17 @@ __global__ void __launch_bounds__(256) regular_p0_tiled_mixed(const RegularArgs 
   }
 }
 
+template <int Q, int OP, bool HELM, bool WIDE>
+__global__ void __launch_bounds__(256) regular_p0_tiled_mixed(const RegularArgs A, const int2* __restrict__ tiles,
+                                                              int symmetric) {
+  __shared__ double2 lds[4][kChunk][kTile];
+  p0_tile_mixed<Q, OP, HELM, WIDE>(A, tiles[blockIdx.x], symmetric, lds);
+}
+
 // FP64 evaluation of a list of near (non-adjacent) P0 x P0 cell pairs, one
 // thread per pair, added to their (unique) entries.
-template <int Q, int OP, bool HELM, bool CENTRED>
-__global__ void __launch_bounds__(256) near_pair_kernel(const RegularArgs A, const int2* __restrict__ pairs,
-                                                        int npairs) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= npairs) return;
-  const int2 pr = pairs[t];
+template <int Q, int OP, bool HELM, bool CENTRED, typename Args>
+__device__ __forceinline__ void near_pair(Args& A, const int2 pr) {
   const int i = pr.x, j = pr.y;
   const double* tg = A.test_geo + (size_t)i * kGeoStride;
   const double* sg = A.trial_geo + (size_t)j * kGeoStride;
@@ -239,6 +267,57 @@ __global__ void __launch_bounds__(256) near_pair_kernel(const RegularArgs A, con
   }
   store_value(A.out, (long long)i + A.ld * (long long)j, {acc.re * kInv4Pi, acc.im * kInv4Pi}, A.out_type,
               kAccumulate);
+}
+
+template <int Q, int OP, bool HELM, bool CENTRED>
+__global__ void __launch_bounds__(256) near_pair_kernel(const RegularArgs A, const int2* __restrict__ pairs,
+                                                        int npairs) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npairs) return;
+  near_pair<Q, OP, HELM, CENTRED>(A, pairs[t]);
+}
+
+// ---------------------------------------------------------------------------
+// Batched mixed precision: persistent launches over the tiles of all items (FP32
+// far field; one launch for centred items, one with the WIDE phase reduction for
+// the others), then the near-pair lists of all items concatenated (item i owns
+// near pairs [first[i], first[i + 1])), one thread per pair.
+// ---------------------------------------------------------------------------
+template <int Q, int OP, bool HELM, bool WIDE, int CH>
+__global__ void __launch_bounds__(256) batched_p0_mixed_persistent(const RegularArgs* __restrict__ items,
+                                                                   const BatchTile* __restrict__ tiles, int ntiles,
+                                                                   int* counter, int symmetric) {
+  __shared__ double2 lds[4][CH][kTile];
+  __shared__ int next;
+  for (;;) {
+    if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+    __syncthreads();
+    const int t = __builtin_amdgcn_readfirstlane(next);
+    __syncthreads();
+    if (t >= ntiles) return;
+    const BatchTile bt = load_uniform(tiles + t);
+    p0_tile_mixed<Q, OP, HELM, WIDE, CH>(constant_view(items)[bt.item], make_int2(bt.a, bt.b), symmetric, lds);
+  }
+}
+
+template <int Q, int OP, bool HELM>
+__global__ void __launch_bounds__(256) batched_near_kernel(const RegularArgs* __restrict__ items,
+                                                           const int* __restrict__ centred,
+                                                           const int2* __restrict__ pairs,
+                                                           const int* __restrict__ first, int nitems) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= first[nitems]) return;
+  int lo = 0, hi = nitems - 1;  // last item with first[item] <= g
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (first[mid] <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  const RegularArgs& A = items[lo];
+  if (HELM && centred[lo])
+    near_pair<Q, OP, HELM, true>(A, pairs[g]);
+  else
+    near_pair<Q, OP, HELM, false>(A, pairs[g]);
 }
 
 }  // namespace dev

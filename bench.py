@@ -55,6 +55,103 @@ def cpu_baseline(points, cells, k, target_s, threads):
     return rows * n / t, rows, t
 
 
+def batch_items(seed, n, dtype, mixed):
+    """The batched workload: n ellipsoids, level uniform in {2..5}, semi-axes uniform in
+    [0.5, 2], k uniform in [1, 10] (numpy default_rng(seed), drawn in that order per item)."""
+    import numpy as np
+
+    from bempp_amd.assembly.boundary import (BoundaryAssemblerOptions, OperatorType, Precision,
+                                             create_helmholtz_assembler)
+    from bempp_amd.element import Continuity, Family, ReferenceCellType, create_family
+    from bempp_amd.function_space import function_space
+    from bempp_amd.shapes import ellipsoid
+
+    rng = np.random.default_rng(seed)
+    items = []
+    for _ in range(n):
+        lev = int(rng.integers(2, 6))
+        axes = rng.uniform(0.5, 2.0, 3)
+        k = float(rng.uniform(1.0, 10.0))
+        if dtype == np.complex64:
+            k = float(np.float32(k))  # a complex64 assembler holds k in FP32; the oracle uses the same k
+        o = BoundaryAssemblerOptions()
+        o.set_regular_quadrature_degree(ReferenceCellType.Triangle, 6)
+        o.set_singular_quadrature_degree(ReferenceCellType.Triangle, ReferenceCellType.Triangle, 4)
+        if mixed:
+            o.set_precision(Precision.Mixed)
+        g = ellipsoid(lev, axes)
+        s = function_space(g, create_family(Family.Lagrange, 0, Continuity.Discontinuous))
+        items.append((g, s, create_helmholtz_assembler(k, OperatorType.SingleLayer, o, dtype=dtype), k, lev))
+    return items
+
+
+def batch_leg(seed, n, steps, mixed):
+    """One persistent (graph-replayed, autotuned) launch for all items, timed, against one
+    assembly launch per item; sample rows of three items checked against the oracle."""
+    import numpy as np
+    import torch
+
+    import oracle_lib as O
+    from bempp_amd.assembly.batch import BatchAssembler
+
+    dtype = np.complex64 if mixed else np.complex128
+    items = batch_items(seed, n, dtype, mixed)
+    entries = sum(len(g.cells) ** 2 for g, *_ in items)
+    t0 = time.perf_counter()
+    b = BatchAssembler([it[2] for it in items], [it[1] for it in items])
+    setup_s = time.perf_counter() - t0
+    tune_ms, _ = b.autotune(None, reps=2)
+    b.set_graph(True)
+    b.assemble()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.assemble()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    # sample rows of the smallest, a median and the largest item against the FP64 oracle
+    order = np.argsort([len(g.cells) for g, *_ in items], kind="stable")
+    errs = []
+    for idx in (order[0], order[len(order) // 2], order[-1]):
+        g, s, a, k, _ = items[idx]
+        rows = min(32, len(g.cells))
+        ref = O.assemble("helmholtz", "single_layer", g.points, g.cells, k=k, rows=(0, rows))
+        got = b.outputs[idx][:rows].cpu().numpy().astype(np.complex128)
+        errs.append(float(np.linalg.norm(got - ref) / np.linalg.norm(ref)))
+    # one launch sequence per item into the same outputs, for comparison
+    for (g, s, a, k, _), o in zip(items, b.outputs):
+        a.assemble_into_device(s, s, o)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for (g, s, a, k, _), o in zip(items, b.outputs):
+            a.assemble_into_device(s, s, o)
+    torch.cuda.synchronize()
+    per_item_ms = (time.perf_counter() - t0) * 1e3 / steps
+    res = {"ms_per_step": ms, "value": entries / (ms * 1e-3), "unit": "entries/s",
+           "per_item_launches_ms": per_item_ms, "speedup_vs_per_item_launches": per_item_ms / ms,
+           "launch_shape": dict(zip(("blocks_per_cu", "chunk", "order"), b.launch())), "autotune_best_ms": tune_ms,
+           "counts": b.counts(), "setup_s": setup_s, "sample_rel_err_vs_oracle": max(errs),
+           "output": "complex64" if mixed else "complex128"}
+    del b, items
+    torch.cuda.empty_cache()
+    return entries, res
+
+
+def copy_calibration():
+    """A plain device copy of 1 GiB (read 1 GiB + write 1 GiB per dispatch), three times:
+    the reference point that scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+
+    n = 1 << 27
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_per_dispatch": n * 8, "dispatches": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,7 +161,16 @@ def main():
     ap.add_argument("--k", type=float, default=5.0)
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-items", type=int, default=64)
+    ap.add_argument("--batch-seed", type=int, default=0)
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the 1 GiB copy used to calibrate the HBM byte counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine assembles on one GPU; multi-GPU runs are not part of this build "
                                    "(sharding by trial-column blocks is a design note only)"}))
@@ -146,6 +252,21 @@ def main():
     flops_reg = n_reg * (36 * FLOP_PER_QPP + FLOP_PER_PAIR_EPILOGUE)
     achieved = flops_reg / (np.mean(reg_ms) * 1e-3) / 1e12
 
+    batched = None
+    if not args.no_batch:
+        del out
+        torch.cuda.empty_cache()
+        batched = {"workload": f"{args.batch_items} Helmholtz SL P0 assemblies: icospheres of level uniform in 2..5 "
+                               f"scaled to ellipsoids (semi-axes uniform in [0.5, 2]), k uniform in [1, 10], "
+                               f"numpy seed {args.batch_seed}"}
+        for mixed in (False, True):
+            e, res = batch_leg(args.batch_seed, args.batch_items, args.batch_steps, mixed)
+            batched["entries"] = e
+            batched["mixed_complex64" if mixed else "fp64_complex128"] = res
+            print(f"[bench] batch {'mixed c64' if mixed else 'fp64 c128'}: {res['ms_per_step']:.2f} ms, per-item "
+                  f"launches {res['per_item_launches_ms']:.2f} ms, sample err {res['sample_rel_err_vs_oracle']:.2e}",
+                  file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
@@ -173,6 +294,7 @@ def main():
                      "basis": f"{n_reg} regular pairs x (36 x {FLOP_PER_QPP} + {FLOP_PER_PAIR_EPILOGUE}) nominal "
                               "flop per launch / mean regular-kernel HIP-event time"},
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(line))
 

@@ -162,7 +162,7 @@ int boundary_assembler_assemble_into_memory(const BoundaryAssemblerT* assembler,
 
 /* Device-resident variant (no counterpart in the reference): `data` is device
  * memory on the current HIP device. accumulate != 0 adds into it, otherwise it
- * is overwritten. `stream` is a hipStream_t (NULL: the assembler's own stream).
+ * is overwritten. `stream` is a hipStream_t (NULL: the device's null stream).
  * The call returns once the work is enqueued; synchronize the stream to wait. */
 int boundary_assembler_assemble_into_device_memory(const BoundaryAssemblerT* assembler, const SpaceT* trial_space,
                                                    const SpaceT* test_space, void* data, int accumulate,
@@ -199,6 +199,45 @@ int boundary_assembler_shape_gradient(const BoundaryAssemblerT* assembler, const
  * milliseconds (ms[0] regular, ms[1] adjacent; -1 when a kernel did not run). */
 int boundary_assembler_set_profiling(BoundaryAssemblerT* assembler, int enable);
 int boundary_assembler_kernel_times(const BoundaryAssemblerT* assembler, float* ms);
+
+/* ---- batched assembly (no counterpart in the reference: it assembles one operator
+ *      per call) ----
+ * n items, item i = assemblers[i] on the P0 space spaces[i] (test = trial), written to
+ * the device buffer outputs[i] (n_i x n_i, Fortran order, the assembler's dtype, on the
+ * current device). Items share kernel, operator, dtype, precision and regular rule;
+ * wavenumbers, grids and singular rules may differ. All items run in one persistent
+ * launch (plus one launch for the adjacent pairs, and one for the near pairs in mixed
+ * precision). The batch keeps device pointers into the grids' caches: free it before
+ * its grids, spaces or assemblers; after grid_set_points on one of its grids,
+ * assembling fails until a new batch is built. NULL on error. */
+typedef struct BoundaryAssemblerBatchT BoundaryAssemblerBatchT;
+BoundaryAssemblerBatchT* boundary_assembler_batch_new(const BoundaryAssemblerT* const* assemblers,
+                                                      const SpaceT* const* spaces, void* const* outputs, size_t n);
+void boundary_assembler_batch_t_free(BoundaryAssemblerBatchT* batch);
+/* Overwrites every output. stream: a hipStream_t (NULL: the device's null stream).
+ * Returns once enqueued. Launches of one batch must not overlap in time (they share
+ * its tile counter): order them on one stream. */
+int boundary_assembler_batch_assemble(BoundaryAssemblerBatchT* batch, void* stream);
+/* enable != 0: capture the launch sequence into a hipGraph on the next assemble and
+ * replay it afterwards (recaptured when the launch shape changes). */
+int boundary_assembler_batch_set_graph(BoundaryAssemblerBatchT* batch, int enable);
+/* Launch shape: blocks_per_cu workgroups per compute unit (0: one workgroup per tile),
+ * trial chunk 4 or 8, tile order 0 (item-major) or 1 (largest items first). */
+int boundary_assembler_batch_set_launch(BoundaryAssemblerBatchT* batch, int blocks_per_cu, int chunk, int order);
+int boundary_assembler_batch_get_launch(const BoundaryAssemblerBatchT* batch, int* blocks_per_cu, int* chunk,
+                                        int* order);
+/* counts[4] = items, pair tiles, adjacent pairs, near pairs */
+int boundary_assembler_batch_counts(const BoundaryAssemblerBatchT* batch, int64_t* counts);
+/* Times candidate launch shapes (best of `reps` launches each; outputs are
+ * overwritten) and keeps the fastest; with cache_path, a JSON-lines file of earlier
+ * results keyed by device and batch signature is read first and appended to.
+ * Returns the best time in ms (the cached one when *from_cache is set), -1 on error. */
+double boundary_assembler_batch_autotune(BoundaryAssemblerBatchT* batch, const char* cache_path, int reps,
+                                         int* from_cache);
+/* Profiling: HIP events around the last eager launch: ms[0] tiles (and near pairs),
+ * ms[1] adjacent pairs; for a graph replay ms[0] is the whole replay and ms[1] -1. */
+int boundary_assembler_batch_set_profiling(BoundaryAssemblerBatchT* batch, int enable);
+int boundary_assembler_batch_kernel_times(const BoundaryAssemblerBatchT* batch, float* ms);
 
 /* ---- matrices: src/bindings/matrices.rs:6-138 ---- */
 size_t matrix_shape0(const MatrixT* matrix);

@@ -154,3 +154,19 @@ def test_no_gpu_fails_loudly():
     o.set_regular_quadrature_degree(ReferenceCellType.Triangle, 6)
     with pytest.raises(ValueError, match="no HIP device"):
         create_laplace_assembler(OperatorType.SingleLayer, o).assemble(s, s)
+
+
+def test_batch_without_gpu_fails_cleanly():
+    """The batched entry point reports an error (no crash, no CPU fallback) without a GPU."""
+    if L.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    g = icosphere(1)
+    s = function_space(g, create_family(Family.Lagrange, 0, Continuity.Discontinuous))
+    a = create_helmholtz_assembler(2.0, OperatorType.SingleLayer)
+    ah = (ctypes.c_void_p * 1)(a._rs_assembler)
+    sh = (ctypes.c_void_p * 1)(s._rs_space)
+    oh = (ctypes.c_void_p * 1)(0x1000)  # never dereferenced: device selection fails first
+    assert not L.lib().boundary_assembler_batch_new(ah, sh, oh, 1)
+    assert L.last_error()
+    assert not L.lib().boundary_assembler_batch_new(ah, sh, oh, 0)
+    assert "at least one" in L.last_error()

@@ -8,6 +8,7 @@ from bempp_amd.assembly.boundary import (BoundaryAssemblerOptions, OperatorType,
                                          create_laplace_assembler)
 from bempp_amd.element import Continuity, Family, ReferenceCellType, create_family
 from bempp_amd.function_space import function_space
+from bempp_amd.grid import Grid
 from bempp_amd.shapes import icosphere, regular_sphere
 
 pytestmark = pytest.mark.gpu
@@ -224,11 +225,21 @@ def test_mixed_precision_p0(kind, k, op):
     assert err > 1e-12  # the FP32 path ran
 
 
-def test_mixed_falls_back_to_fp64():
-    """Coarse mesh with k = 5 (|k| (rad_i + rad_j) > 0.5): the mixed option falls back to FP64."""
-    g = icosphere(2)
+@pytest.mark.parametrize("op", list(OPS))
+def test_mixed_precision_wide_phase(op):
+    """Coarse stretched mesh with a large wavenumber (level-3 icosphere scaled to semi-axes
+    0.7, 1.3, 1.8; k = 8, so |k| (rad_i + rad_j) > 0.5 and the FP32 phase k (r - R) needs
+    its own range reduction): relative Frobenius error <= 1e-6 against the FP64 oracle."""
+    g0 = icosphere(3)
+    pts = g0.points * np.array([0.7, 1.3, 1.8])
+    g = Grid(pts, g0.cells)
     s = _space(g, "dp0")
-    a = _assembler("helmholtz", "single_layer", 5.0, _mixed_options())
+    a = _assembler("helmholtz", op, 8.0, _mixed_options())
+    print(f"[mixed-wide] {op}", flush=True)
     A = a.assemble(s, s)
-    R = O.assemble("helmholtz", "single_layer", g.points, g.cells, k=5.0)
-    assert _rel(A, R) < TOL
+    R = O.assemble("helmholtz", op, pts, g0.cells, k=8.0)
+    err = _rel(A, R)
+    emax = np.abs(A - R).max() / np.abs(R).max()
+    print(f"[mixed-wide]   rel Frobenius err {err:.3e}, max entry err {emax:.3e}", flush=True)
+    assert err <= 1e-6 and emax <= 1e-5
+    assert err > 1e-12
