@@ -57,7 +57,7 @@ struct Batch {
   int* centred = nullptr;
   dev::BatchTile* tiles[2] = {nullptr, nullptr};  // tile orders: item-major, largest items first
   int ntiles = 0;
-  int ngroup[2] = {0, 0};  // tiles of non-centred items (first), of centred items
+  int ngroup[3] = {0, 0, 0};  // tiles of the items of phase tier 0, 1, 2 (in this order)
   int* counter = nullptr;
   dev::SingularArgs* sitems = nullptr;
   int* small = nullptr;
@@ -109,25 +109,35 @@ void drop_graph(Batch& b) {
 // ---- launches ----
 template <int Q, int OP, bool HELM, int CH>
 void launch_tiles_t(const Batch& b, int blocks, hipStream_t s) {
-  // group 0: items without the centred phase (WIDE phase reduction in mixed precision),
-  // group 1: centred items (Helmholtz only)
+  // group g = phase tier of the items: 0 not centred (range-reduced phase), 1 and 2
+  // centred small-argument tiers (Helmholtz only)
   const int sym = b.symmetric ? 1 : 0;
-  for (int g = 0; g < 2; ++g) {
+  int first = 0;
+  for (int g = 0; g < 3; ++g) {
     const int nt = b.ngroup[g];
+    const dev::BatchTile* tg = b.tiles[b.order] + first;
+    first += nt;
     if (nt == 0) continue;
     const int grid = blocks > 0 ? std::min(nt, blocks) : nt;
-    const dev::BatchTile* tg = b.tiles[b.order] + (g ? b.ngroup[0] : 0);
     int* ctr = b.counter + g;
     if (b.mixed) {
-      if (g == 1)
-        dev::batched_p0_mixed_persistent<Q, OP, HELM, false, CH><<<grid, 256, 0, s>>>(b.items, tg, nt, ctr, sym);
+      if (!HELM)
+        dev::batched_p0_mixed_persistent<Q, OP, HELM, 1, CH><<<grid, 256, 0, s>>>(b.items, tg, nt, ctr, sym);
+      else if (g == 2)
+        dev::batched_p0_mixed_persistent<Q, OP, HELM, 2, CH><<<grid, 256, 0, s>>>(b.items, tg, nt, ctr, sym);
+      else if (g == 1)
+        dev::batched_p0_mixed_persistent<Q, OP, HELM, 1, CH><<<grid, 256, 0, s>>>(b.items, tg, nt, ctr, sym);
       else
-        dev::batched_p0_mixed_persistent<Q, OP, HELM, HELM, CH><<<grid, 256, 0, s>>>(b.items, tg, nt, ctr, sym);
+        dev::batched_p0_mixed_persistent<Q, OP, HELM, 0, CH><<<grid, 256, 0, s>>>(b.items, tg, nt, ctr, sym);
     } else {
-      if (g == 1)
-        dev::batched_p0_persistent<Q, OP, HELM, HELM, CH><<<grid, 256, 0, s>>>(b.items, tg, nt, ctr, sym);
+      if (!HELM)
+        dev::batched_p0_persistent<Q, OP, HELM, 0, CH><<<grid, 256, 0, s>>>(b.items, tg, nt, ctr, sym);
+      else if (g == 2)
+        dev::batched_p0_persistent<Q, OP, HELM, 2, CH><<<grid, 256, 0, s>>>(b.items, tg, nt, ctr, sym);
+      else if (g == 1)
+        dev::batched_p0_persistent<Q, OP, HELM, 1, CH><<<grid, 256, 0, s>>>(b.items, tg, nt, ctr, sym);
       else
-        dev::batched_p0_persistent<Q, OP, HELM, false, CH><<<grid, 256, 0, s>>>(b.items, tg, nt, ctr, sym);
+        dev::batched_p0_persistent<Q, OP, HELM, 0, CH><<<grid, 256, 0, s>>>(b.items, tg, nt, ctr, sym);
     }
   }
 }
@@ -166,7 +176,7 @@ void launch_all_t(const Batch& b, hipStream_t s, bool events) {
   const int blocks = b.blocks_per_cu * b.cus;  // 0: one workgroup per tile
   if (events) HIP_CHECK_B(hipEventRecord(b.ev[0], s));
   if (b.ntiles > 0) {
-    dev::reset_counter_kernel<int><<<1, 64, 0, s>>>(b.counter, 2);
+    dev::reset_counter_kernel<int><<<1, 64, 0, s>>>(b.counter, 3);
     launch_tiles_q<OP, HELM>(b, blocks, s);
   }
   if (b.mixed && b.nnear > 0) launch_near_q<OP, HELM>(b, s);
@@ -251,7 +261,7 @@ Batch* batch_new(const std::vector<const Assembler*>& asms, const std::vector<co
     if (P.Q != b->Q) throw Error("batch items must share the regular quadrature degree");
     R[i] = P.R;
     R[i].mode = dev::kStore;
-    centred[i] = P.centred ? 1 : 0;
+    centred[i] = P.tier;  // 0: not centred; 1, 2: phase tiers (nonzero = centred)
     small[i] = P.small_adjacent ? 1 : 0;
     S[i] = P.S;
     S[i].mode = dev::kAccumulate;
@@ -276,16 +286,16 @@ Batch* batch_new(const std::vector<const Assembler*>& asms, const std::vector<co
     for (int bb = 0; bb < nt; ++bb)
       for (int aa = 0; aa < (b->symmetric ? bb + 1 : nt); ++aa) t.push_back({item, aa, bb, 0});
   };
-  for (int g = 0; g < 2; ++g) {
+  for (int g = 0; g < 3; ++g) {
+    const size_t before = t0.size();
     for (int i = 0; i < b->n; ++i)
       if (centred[i] == g) add_tiles(t0, i);
     for (auto& so : size_order)
       if (centred[so.second] == g) add_tiles(t1, so.second);
-    if (g == 0) b->ngroup[0] = (int)t0.size();
+    b->ngroup[g] = (int)(t0.size() - before);
   }
   if (t0.size() > (size_t)INT32_MAX - 65536) throw Error("too many tiles in the batch");
   b->ntiles = (int)t0.size();
-  b->ngroup[1] = b->ntiles - b->ngroup[0];
   b->npairs = sfirst[n];
   b->nnear = (int)near_all.size();
   try {
@@ -293,7 +303,7 @@ Batch* batch_new(const std::vector<const Assembler*>& asms, const std::vector<co
     b->centred = upload_b(centred);
     b->tiles[0] = upload_b(t0);
     b->tiles[1] = upload_b(t1);
-    b->counter = upload_b(std::vector<int>(2, 0));
+    b->counter = upload_b(std::vector<int>(3, 0));
     b->sitems = upload_b(S);
     b->small = upload_b(small);
     b->sfirst = upload_b(sfirst);

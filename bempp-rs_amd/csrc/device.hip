@@ -411,11 +411,14 @@ const int2* tile_list(int device, int nta, int ntb, bool symmetric, int* count) 
 }
 
 // ---- kernel dispatch: tiled P0 x P0 or general, by operator, kernel and phase mode ----
-template <int OP, bool HELM, bool CENTRED>
+// CENTRED: phase tier (0: full reduction, 1: |k (r - R)| <= 0.5, 2: <= 0.25; tier 2
+// only for the P0 x P0 tile kernel, the general kernel runs tier 1 there).
+template <int OP, bool HELM, int CENTRED>
 void launch_q(bool tiled, int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& A, const int2* tiles, int sym) {
+  constexpr int GC = CENTRED > 1 ? 1 : CENTRED;
 #define BEMPP_LAUNCH(QQ)                                                                 \
   if (tiled) dev::regular_p0_tiled<QQ, OP, HELM, CENTRED><<<grid, 256, 0, s>>>(A, tiles, sym); \
-  else dev::regular_kernel<QQ, OP, HELM, CENTRED><<<grid, 256, 0, s>>>(A);
+  else dev::regular_kernel<QQ, OP, HELM, GC><<<grid, 256, 0, s>>>(A);
   switch (Q) {
     case 1: BEMPP_LAUNCH(1); break;
     case 3: BEMPP_LAUNCH(3); break;
@@ -426,7 +429,7 @@ void launch_q(bool tiled, int Q, dim3 grid, hipStream_t s, const dev::RegularArg
 #undef BEMPP_LAUNCH
 }
 
-template <bool HELM, bool CENTRED>
+template <bool HELM, int CENTRED>
 void launch_op(int op, bool tiled, int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& A, const int2* tiles,
                int sym) {
   switch (op) {
@@ -440,23 +443,26 @@ void launch_op(int op, bool tiled, int Q, dim3 grid, hipStream_t s, const dev::R
 
 // Regular pairs: `tiled` selects the P0 x P0 tile kernel (grid = tile count),
 // otherwise the general kernel (grid = test blocks x trial chunks).
-void launch_regular(int op, bool helm, bool centred, bool tiled, int Q, dim3 grid, hipStream_t s,
+void launch_regular(int op, bool helm, int tier, bool tiled, int Q, dim3 grid, hipStream_t s,
                     const dev::RegularArgs& A, const int2* tiles = nullptr, int sym = 0) {
-  if (!helm) launch_op<false, false>(op, tiled, Q, grid, s, A, tiles, sym);
-  else if (centred) launch_op<true, true>(op, tiled, Q, grid, s, A, tiles, sym);
-  else launch_op<true, false>(op, tiled, Q, grid, s, A, tiles, sym);
+  if (!helm) launch_op<false, 0>(op, tiled, Q, grid, s, A, tiles, sym);
+  else if (tier == 2 && tiled) launch_op<true, 2>(op, tiled, Q, grid, s, A, tiles, sym);
+  else if (tier >= 1) launch_op<true, 1>(op, tiled, Q, grid, s, A, tiles, sym);
+  else launch_op<true, 0>(op, tiled, Q, grid, s, A, tiles, sym);
   HIP_CHECK(hipGetLastError());
 }
 
 // Mixed precision (P0 x P0, one grid): FP32 far-pair tiles, then the FP64 near-pair list.
-template <int OP, bool HELM, bool CENTRED>
+// TIER: phase tier of the cell pairs (0: range-reduced FP32 phase and FP64 near pairs
+// without centring; 1, 2: centred small-argument tiers).
+template <int OP, bool HELM, int TIER>
 void launch_mixed_q(int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& A, const int2* tiles, int sym,
                     const int2* near, int nnear) {
   dim3 ng((unsigned)((nnear + 255) / 256));
+  constexpr int NC = HELM && TIER > 0 ? 1 : 0;  // the near-pair kernel runs tier 1 when centred
 #define BEMPP_LAUNCH(QQ)                                                                      \
-  if (CENTRED || !HELM) dev::regular_p0_tiled_mixed<QQ, OP, HELM, false><<<grid, 256, 0, s>>>(A, tiles, sym); \
-  else dev::regular_p0_tiled_mixed<QQ, OP, HELM, true><<<grid, 256, 0, s>>>(A, tiles, sym);            \
-  if (nnear > 0) dev::near_pair_kernel<QQ, OP, HELM, CENTRED><<<ng, 256, 0, s>>>(A, near, nnear);
+  dev::regular_p0_tiled_mixed<QQ, OP, HELM, TIER><<<grid, 256, 0, s>>>(A, tiles, sym);        \
+  if (nnear > 0) dev::near_pair_kernel<QQ, OP, HELM, NC><<<ng, 256, 0, s>>>(A, near, nnear);
   switch (Q) {
     case 1: BEMPP_LAUNCH(1); break;
     case 3: BEMPP_LAUNCH(3); break;
@@ -467,23 +473,24 @@ void launch_mixed_q(int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& A, 
 #undef BEMPP_LAUNCH
 }
 
-template <bool HELM, bool CENTRED>
+template <bool HELM, int TIER>
 void launch_mixed_op(int op, int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& A, const int2* tiles, int sym,
                      const int2* near, int nnear) {
   switch (op) {
-    case kSL: launch_mixed_q<kSL, HELM, CENTRED>(Q, grid, s, A, tiles, sym, near, nnear); break;
-    case kDL: launch_mixed_q<kDL, HELM, CENTRED>(Q, grid, s, A, tiles, sym, near, nnear); break;
-    case kADL: launch_mixed_q<kADL, HELM, CENTRED>(Q, grid, s, A, tiles, sym, near, nnear); break;
-    case kHS: launch_mixed_q<kHS, HELM, CENTRED>(Q, grid, s, A, tiles, sym, near, nnear); break;
+    case kSL: launch_mixed_q<kSL, HELM, TIER>(Q, grid, s, A, tiles, sym, near, nnear); break;
+    case kDL: launch_mixed_q<kDL, HELM, TIER>(Q, grid, s, A, tiles, sym, near, nnear); break;
+    case kADL: launch_mixed_q<kADL, HELM, TIER>(Q, grid, s, A, tiles, sym, near, nnear); break;
+    case kHS: launch_mixed_q<kHS, HELM, TIER>(Q, grid, s, A, tiles, sym, near, nnear); break;
     default: throw Error("operator not supported by the dense assembler");
   }
 }
 
-void launch_mixed(int op, bool helm, bool centred, int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& A,
+void launch_mixed(int op, bool helm, int tier, int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& A,
                   const int2* tiles, int sym, const int2* near, int nnear) {
-  if (!helm) launch_mixed_op<false, false>(op, Q, grid, s, A, tiles, sym, near, nnear);
-  else if (centred) launch_mixed_op<true, true>(op, Q, grid, s, A, tiles, sym, near, nnear);
-  else launch_mixed_op<true, false>(op, Q, grid, s, A, tiles, sym, near, nnear);
+  if (!helm) launch_mixed_op<false, 1>(op, Q, grid, s, A, tiles, sym, near, nnear);
+  else if (tier == 2) launch_mixed_op<true, 2>(op, Q, grid, s, A, tiles, sym, near, nnear);
+  else if (tier == 1) launch_mixed_op<true, 1>(op, Q, grid, s, A, tiles, sym, near, nnear);
+  else launch_mixed_op<true, 0>(op, Q, grid, s, A, tiles, sym, near, nnear);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -558,7 +565,9 @@ Prepared prepare(const Assembler& a, const FunctionSpace& trial_c, const Functio
   DeviceGridCache* sc = P.same_grid ? tc : grid_cache(sgrid, P.device);
   // |k (r - R)| <= |k| (rad_test + rad_trial) for every quadrature-point pair of a
   // regular cell pair; the small-argument sine/cosine needs this bound <= 0.5.
-  P.centred = a.kind == kHelmholtz && std::fabs(a.k) * (tc->max_radius + sc->max_radius) <= 0.5;
+  const double phase_bound = std::fabs(a.k) * (tc->max_radius + sc->max_radius);
+  P.centred = a.kind == kHelmholtz && phase_bound <= 0.5;
+  P.tier = !P.centred ? 0 : (phase_bound <= 0.25 ? 2 : 1);
   // points of two cells sharing a vertex are at most 2 (rad_i + rad_j) apart
   P.small_adjacent = a.kind == kHelmholtz && P.same_grid && std::fabs(a.k) * 4.0 * tc->max_radius <= 0.5;
   dev::RegularArgs& R = P.R;
@@ -687,14 +696,14 @@ void assemble_device(const Assembler& a, const FunctionSpace& trial, const Funct
       int ntiles = 0;
       const int2* tiles = tile_list(P.device, nta, ntb, sym, &ntiles);
       if (P.mixed) {
-        launch_mixed(a.op, a.kind == kHelmholtz, P.centred, P.Q, dim3(ntiles), P.stream, R, tiles, sym ? 1 : 0,
+        launch_mixed(a.op, a.kind == kHelmholtz, P.tier, P.Q, dim3(ntiles), P.stream, R, tiles, sym ? 1 : 0,
                      P.near, P.nnear);
       } else {
-        launch_regular(a.op, a.kind == kHelmholtz, P.centred, true, P.Q, dim3(ntiles), P.stream, R, tiles,
+        launch_regular(a.op, a.kind == kHelmholtz, P.tier, true, P.Q, dim3(ntiles), P.stream, R, tiles,
                        sym ? 1 : 0);
       }
     } else {
-      launch_regular(a.op, a.kind == kHelmholtz, P.centred, false, P.Q, grid, P.stream, R);
+      launch_regular(a.op, a.kind == kHelmholtz, P.tier, false, P.Q, grid, P.stream, R);
     }
     if (a.profiling) {
       HIP_CHECK(hipEventRecord((hipEvent_t)am.events[1], P.stream));

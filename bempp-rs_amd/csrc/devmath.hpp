@@ -1,7 +1,9 @@
 // devmath.hpp -- FP64 reciprocal square root and sine/cosine for the pair kernels.
 //
 // rsqrt: the hardware estimate v_rsq_f64 (measured relative error 5.2e-8 on
-//   log-uniform inputs in [1e-8, 1e8]) refined by one third-order step.
+//   log-uniform inputs in [1e-8, 1e8]) refined by one third-order step (rsqrt_nr,
+//   exact to rounding) or one Newton step (rsqrt_fast, relative error below 4.1e-15,
+//   always from below; used per quadrature-point pair of regular cell pairs).
 // sincos: argument reduction x = n pi/2 + r with |r| <= pi/4 by the three-term
 //   Cody-Waite scheme (W. J. Cody, W. Waite, Software Manual for the Elementary
 //   Functions, 1980) evaluated with fused multiply-adds against pi/2 split into
@@ -23,6 +25,13 @@ __device__ __forceinline__ double rsqrt_nr(double x) {
   const double e = fma(-x * y, y, 1.0);
   const double t = fma(0.375, e, 0.5);
   return fma(y * e, t, y);
+}
+
+// One Newton step: y (1 + e/2) = 1/sqrt(x) (1 - 3e^2/8 + O(e^3)), |e| <= 1.1e-7.
+__device__ __forceinline__ double rsqrt_fast(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(-x * y, y, 1.0);
+  return fma(y * e, 0.5, y);
 }
 
 // Valid for |x| <= 1e8 (n = rint(2x/pi) fits an int and every reduction step is
@@ -68,27 +77,40 @@ __device__ __forceinline__ void sincos_cw(double x, double* s_out, double* c_out
   *c_out = __longlong_as_double(__double_as_longlong(cc) ^ sgn_c);
 }
 
-// sin and cos of a small argument |x| <= 0.5 without reduction: Taylor series to
-// x^13 (sin) and x^14 (cos); truncation below 5e-17 and 1e-18 relative on [-0.5, 0.5].
-// Used for the phase k (r - R) of a quadrature-point pair relative to its cell pair's
-// centroid distance R; the factor e^{ikR} is applied once per cell pair.
+// sin and cos of a small argument without reduction, for the phase k (r - R) of a
+// quadrature-point pair relative to its cell pair's centroid distance R (the factor
+// e^{ikR} is applied once per cell pair). TIER 1: |x| <= 0.5, TIER 2: |x| <= 0.25.
+//   sin x = x + x^3 P(x^2),  cos x = 1 + x^2 C(x^2)
+// with P, C the weighted minimax polynomials (absolute error of sin/cos) of degrees
+// 4/5 (tier 1) and 3/4 (tier 2), computed by Lawson's iteration at 50 digits; the
+// truncation errors are below 1e-17 and the float64 Horner evaluation errors below
+// 6e-17 (sin) and 1.2e-16 (cos) on the whole interval.
+template <int TIER = 1>
 __device__ __forceinline__ void sincos_small(double x, double* s_out, double* c_out) {
-  const double x2 = x * x;
-  double ps = 1.6059043836821613e-10;
-  ps = fma(ps, x2, -2.505210838544172e-08);
-  ps = fma(ps, x2, 2.7557319223985893e-06);
-  ps = fma(ps, x2, -0.0001984126984126984);
-  ps = fma(ps, x2, 0.008333333333333333);
-  ps = fma(ps, x2, -0.16666666666666666);
-  *s_out = fma(ps * x2, x, x);
-  double pc = -1.1470745597729725e-11;
-  pc = fma(pc, x2, 2.08767569878681e-09);
-  pc = fma(pc, x2, -2.755731922398589e-07);
-  pc = fma(pc, x2, 2.48015873015873e-05);
-  pc = fma(pc, x2, -0.001388888888888889);
-  pc = fma(pc, x2, 0.041666666666666664);
-  pc = fma(pc, x2, -0.5);
-  *c_out = fma(pc, x2, 1.0);
+  static_assert(TIER == 1 || TIER == 2, "phase tiers: 1 (|x| <= 0.5), 2 (|x| <= 0.25)");
+  const double u = x * x;
+  double p, c;
+  if constexpr (TIER == 1) {
+    p = fma(u, -2.4927410039626402e-08, 2.7556952897882737e-06);
+    p = fma(p, u, -0.00019841269341158048);
+    p = fma(p, u, 0.008333333333023885);
+    p = fma(p, u, -0.16666666666666);
+    c = fma(u, 2.077788258235537e-09, -2.7556985499996286e-07);
+    c = fma(c, u, 2.480158674730518e-05);
+    c = fma(c, u, -0.00138888888884234);
+    c = fma(c, u, 0.04166666666666487);
+    c = fma(c, u, -0.5);
+  } else {
+    p = fma(u, 2.751683000751954e-06, -0.0001984124679978689);
+    p = fma(p, u, 0.00833333332798428);
+    p = fma(p, u, -0.1666666666666255);
+    c = fma(u, -2.751887867547197e-07, 2.4801560959230762e-05);
+    c = fma(c, u, -0.0013888888880693657);
+    c = fma(c, u, 0.04166666666665554);
+    c = fma(c, u, -0.49999999999999994);
+  }
+  *s_out = fma(x * u, p, x);
+  *c_out = fma(u, c, 1.0);
 }
 
 }  // namespace dev

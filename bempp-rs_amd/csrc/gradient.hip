@@ -248,9 +248,9 @@ __global__ void __launch_bounds__(256) apply_regular_kernel(const GradArgs G, do
     if constexpr (CENTRED) {
       double kR;
       const cplx e = pair_phase(ct, sg + kGeoCentroid, A.k, &kR);
-      v = cmul(p0_pair<Q, OP, HELM, true>(xp, wp, nx, sp, ny, A.k, kR), e);
+      v = cmul(p0_pair<Q, OP, HELM, CENTRED>(xp, wp, nx, sp, ny, A.k, kR), e);
     } else {
-      v = p0_pair<Q, OP, HELM, false>(xp, wp, nx, sp, ny, A.k, 0.0);
+      v = p0_pair<Q, OP, HELM, 0>(xp, wp, nx, sp, ny, A.k, 0.0);
     }
     if constexpr (OP == 3) {
       const double nn = HELM ? -A.k * A.k * (nx[0] * ny[0] + nx[1] * ny[1] + nx[2] * ny[2]) : 0.0;

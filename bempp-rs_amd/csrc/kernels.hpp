@@ -80,20 +80,21 @@ struct cplx {
 };
 
 // Kernel scalar (without the 1/(4 pi) factor) for displacement d = x - y.
-// nx: test normal, ny: trial normal. CENTRED (Helmholtz): the phase is taken
-// relative to kR, i.e. the result lacks the factor e^{ikR} that the caller
-// applies once per cell pair; requires |k r - kR| <= 0.5.
-template <int OP, bool HELM, bool CENTRED = false>
+// nx: test normal, ny: trial normal. CENTRED (Helmholtz, phase tier 1 or 2): the
+// phase is taken relative to kR, i.e. the result lacks the factor e^{ikR} that the
+// caller applies once per cell pair; requires |k r - kR| <= 0.5 (tier 1) or <= 0.25
+// (tier 2). CENTRED = 0: full argument reduction.
+template <int OP, bool HELM, int CENTRED = 0>
 __device__ __forceinline__ cplx kernel_scalar(double dx, double dy, double dz, double k, const double* nx,
                                               const double* ny, double kR = 0.0) {
   const double r2 = dx * dx + dy * dy + dz * dz;
-  const double rinv = rsqrt_nr(r2);
+  const double rinv = rsqrt_fast(r2);
   cplx g;
   double kr = 0.0, s = 0.0, c = 1.0;
   if constexpr (HELM) {
     const double r = r2 * rinv;
     if constexpr (CENTRED) {
-      sincos_small(fma(k, r, -kR), &s, &c);
+      sincos_small<CENTRED>(fma(k, r, -kR), &s, &c);
       if constexpr (OP == 1 || OP == 2) kr = k * r;
     } else {
       kr = k * r;
@@ -118,6 +119,28 @@ __device__ __forceinline__ cplx kernel_scalar(double dx, double dy, double dz, d
     } else {
       return {f, 0.0};
     }
+  }
+}
+
+// acc += w G for the single-layer kernel (without 1/(4 pi)); the weight is folded into
+// 1/r so that the complex update costs one product and two fused multiply-adds.
+template <bool HELM, int CENTRED>
+__device__ __forceinline__ void green_accumulate(double dx, double dy, double dz, double k, double kR, double w,
+                                                 cplx& acc) {
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  const double rinv = rsqrt_fast(r2);
+  if constexpr (HELM) {
+    const double r = r2 * rinv;
+    double s, c;
+    if constexpr (CENTRED)
+      sincos_small<CENTRED>(fma(k, r, -kR), &s, &c);
+    else
+      sincos_cw(k * r, &s, &c);
+    const double rw = rinv * w;
+    acc.re = fma(c, rw, acc.re);
+    acc.im = fma(s, rw, acc.im);
+  } else {
+    acc.re = fma(rinv, w, acc.re);
   }
 }
 
@@ -334,7 +357,7 @@ __global__ void __launch_bounds__(256) regular_kernel(const RegularArgs A) {
 constexpr int kTile = 64;
 constexpr int kChunk = 8;
 
-template <int Q, int OP, bool HELM, bool CENTRED>
+template <int Q, int OP, bool HELM, int CENTRED>
 __device__ __forceinline__ cplx p0_pair(const double (&xp)[Q][3], const double (&wp)[Q], const double* nx,
                                         const double (&sp)[Q * 4], const double* ny, double k, double kR) {
   cplx acc = {0.0, 0.0};
@@ -343,10 +366,15 @@ __device__ __forceinline__ cplx p0_pair(const double (&xp)[Q][3], const double (
     cplx ap = {0.0, 0.0};
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      const cplx kv = kernel_scalar<OP, HELM, CENTRED>(xp[p][0] - sp[4 * q], xp[p][1] - sp[4 * q + 1],
-                                                       xp[p][2] - sp[4 * q + 2], k, nx, ny, kR);
-      ap.re = fma(kv.re, sp[4 * q + 3], ap.re);
-      ap.im = fma(kv.im, sp[4 * q + 3], ap.im);
+      if constexpr (OP == 0 || OP == 3) {  // single layer (and the P0 hypersingular n.n term)
+        green_accumulate<HELM, CENTRED>(xp[p][0] - sp[4 * q], xp[p][1] - sp[4 * q + 1], xp[p][2] - sp[4 * q + 2], k,
+                                        kR, sp[4 * q + 3], ap);
+      } else {
+        const cplx kv = kernel_scalar<OP, HELM, CENTRED>(xp[p][0] - sp[4 * q], xp[p][1] - sp[4 * q + 1],
+                                                         xp[p][2] - sp[4 * q + 2], k, nx, ny, kR);
+        ap.re = fma(kv.re, sp[4 * q + 3], ap.re);
+        ap.im = fma(kv.im, sp[4 * q + 3], ap.im);
+      }
     }
     acc.re = fma(ap.re, wp[p], acc.re);
     acc.im = fma(ap.im, wp[p], acc.im);
@@ -358,7 +386,7 @@ __device__ __forceinline__ cplx p0_pair(const double (&xp)[Q][3], const double (
 // the mirror transpose stores CH contiguous entries per segment.
 // Args: RegularArgs, possibly qualified with the constant address space (batched
 // launches read each item's arguments in place, with scalar loads).
-template <int Q, int OP, bool HELM, bool CENTRED, int CH = kChunk, typename Args>
+template <int Q, int OP, bool HELM, int CENTRED, int CH = kChunk, typename Args>
 __device__ __forceinline__ void p0_tile(Args& A, const int2 tile, int symmetric,
                                         double2 (&lds)[4][CH][kTile]) {
   const int lane = threadIdx.x & 63;
@@ -409,9 +437,9 @@ __device__ __forceinline__ void p0_tile(Args& A, const int2 tile, int symmetric,
           const double cs[3] = {sg[kGeoCentroid], sg[kGeoCentroid + 1], sg[kGeoCentroid + 2]};
           double kR;
           const cplx e = pair_phase(ct, cs, A.k, &kR);
-          acc = cmul(p0_pair<Q, OP, HELM, true>(xp, wp, nx, sp, ny, A.k, kR), e);
+          acc = cmul(p0_pair<Q, OP, HELM, CENTRED>(xp, wp, nx, sp, ny, A.k, kR), e);
         } else {
-          acc = p0_pair<Q, OP, HELM, false>(xp, wp, nx, sp, ny, A.k, 0.0);
+          acc = p0_pair<Q, OP, HELM, 0>(xp, wp, nx, sp, ny, A.k, 0.0);
         }
       }
       if constexpr (OP == 3) {  // P0 hypersingular: only the Helmholtz -k^2 n_x.n_y term survives
@@ -440,7 +468,7 @@ __device__ __forceinline__ void p0_tile(Args& A, const int2 tile, int symmetric,
   }
 }
 
-template <int Q, int OP, bool HELM, bool CENTRED>
+template <int Q, int OP, bool HELM, int CENTRED>
 __global__ void __launch_bounds__(256) regular_p0_tiled(const RegularArgs A, const int2* __restrict__ tiles,
                                                         int symmetric) {
   __shared__ double2 lds[4][kChunk][kTile];
@@ -460,8 +488,8 @@ struct BatchTile {
   int item, a, b, pad;
 };
 
-template <int Q, int OP, bool HELM, bool CENTRED, int CH>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) batched_p0_persistent(const RegularArgs* __restrict__ items,
+template <int Q, int OP, bool HELM, int CENTRED, int CH>
+__global__ void __launch_bounds__(256) batched_p0_persistent(const RegularArgs* __restrict__ items,
                                                              const BatchTile* __restrict__ tiles, int ntiles,
                                                              int* counter, int symmetric) {
   __shared__ double2 lds[4][CH][kTile];
@@ -658,6 +686,7 @@ struct FunctionSpace;  // core.hpp
 struct Prepared {
   int device;
   bool centred;         // Helmholtz phases taken relative to the cell-pair centroid distance
+  int tier;             // 0: not centred; 1: |k| (rad_i + rad_j) <= 0.5; 2: <= 0.25
   bool small_adjacent;  // |k| |x - y| <= 0.5 on every adjacent cell pair
   bool mixed;           // FP32 far pairs + FP64 near-pair list (P0 x P0 on one grid)
   const int2* near;

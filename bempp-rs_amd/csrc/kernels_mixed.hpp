@@ -7,8 +7,8 @@
 //   cell centroid (stored in FP32) and e = o_p - o_q:
 //     s = e . (2D + e) = r^2 - R^2        (no cancellation: |e| << R)
 //     r^2 = R^2 + s, 1/r = rsq(r^2)
-//     delta = r - R = s / (r + R),  phase k delta (|k delta| <= 0.5: short series;
-//     WIDE: any |k delta|, reduced by multiples of pi/2 in FP32 first)
+//     delta = r - R = s / (r + R),  phase k delta (|k delta| <= 0.5 or 0.25: short
+//     minimax series; otherwise reduced by multiples of pi/2 in FP32 first)
 //   the 6 trial points of a test point are summed in FP32, the rows in FP64, and the
 //   cell-pair sum is rotated by e^{ikR} in FP64.
 // Far means R >= kNearFactor (rad_test + rad_trial): then |e| <= R/3, r >= 2R/3 and the
@@ -38,16 +38,28 @@ struct cplxf {
   float re, im;
 };
 
-// sin/cos for |x| <= 0.5 in FP32: Taylor to x^7 / x^8 (truncation below 1e-8).
+// sin/cos in FP32 without reduction: TIER 1 for |x| <= 0.5, TIER 2 for |x| <= 0.25.
+// sin x = x + x^3 P(x^2), cos x = 1 + x^2 C(x^2) with weighted minimax P, C of degrees
+// 2/3 (tier 1) and 1/2 (tier 2) (Lawson's iteration at 50 digits); float32 Horner
+// errors below 1.8e-8 (sin) and 3.7e-8 (cos).
+template <int TIER>
 __device__ __forceinline__ void sincos_small_f32(float x, float* s, float* c) {
-  const float x2 = x * x;
-  float ps = fmaf(x2, -1.9841270e-4f, 8.3333333e-3f);
-  ps = fmaf(ps, x2, -1.6666667e-1f);
-  *s = fmaf(ps * x2, x, x);
-  float pc = fmaf(x2, 2.4801587e-5f, -1.3888889e-3f);
-  pc = fmaf(pc, x2, 4.1666667e-2f);
-  pc = fmaf(pc, x2, -0.5f);
-  *c = fmaf(pc, x2, 1.0f);
+  static_assert(TIER == 1 || TIER == 2, "phase tiers: 1 (|x| <= 0.5), 2 (|x| <= 0.25)");
+  const float u = x * x;
+  float p, q;
+  if constexpr (TIER == 1) {
+    p = fmaf(u, -1.9700505e-4f, 8.3331093e-3f);
+    p = fmaf(p, u, -1.6666666e-1f);
+    q = fmaf(u, 2.4634197e-5f, -1.3888538e-3f);
+    q = fmaf(q, u, 4.1666664e-2f);
+    q = fmaf(q, u, -0.5f);
+  } else {
+    p = fmaf(u, 8.3148913e-3f, -1.6666627e-1f);
+    q = fmaf(u, -1.3859142e-3f, 4.1666559e-2f);
+    q = fmaf(q, u, -0.5f);
+  }
+  *s = fmaf(x * u, p, x);
+  *c = fmaf(u, q, 1.0f);
 }
 
 // sin/cos in FP32 for moderate |x| (the far-pair phase k delta, |k delta| <= |k| (rad_i + rad_j)):
@@ -67,56 +79,60 @@ __device__ __forceinline__ void sincos_f32_wide(float x, float* s, float* c) {
   pc = fmaf(pc, t2, 4.1666667e-2f);
   pc = fmaf(pc, t2, -0.5f);
   const float ct = fmaf(pc, t2, 1.0f);
-  const int q = (int)n & 3;
-  const float sv = (q & 1) ? ct : st, cv = (q & 1) ? st : ct;
-  *s = (q & 2) ? -sv : sv;
-  *c = ((q + 1) & 2) ? -cv : cv;
+  // quadrant q = n mod 4: sin x = (s, c, -s, -c)[q], cos x = (c, -s, -c, s)[q]; the
+  // negations flip the sign bit (bit 1 of q for sin, of q + 1 for cos)
+  const int q = (int)n;
+  const bool swap = q & 1;
+  const float sv = swap ? ct : st, cv = swap ? st : ct;
+  *s = __int_as_float(__float_as_int(sv) ^ ((q & 2) << 30));
+  *c = __int_as_float(__float_as_int(cv) ^ (((q + 1) & 2) << 30));
 }
 
-// FP32 kernel scalar of one far quadrature-point pair, without e^{ikR} and 1/(4 pi).
-template <int OP, bool HELM, bool WIDE>
-__device__ __forceinline__ cplxf kernel_scalar_f32(float ex, float ey, float ez, const float (&D)[3], float R2,
-                                                   float R, float k, const float* nx, const float* ny) {
-  const float fx = fmaf(2.0f, D[0], ex), fy = fmaf(2.0f, D[1], ey), fz = fmaf(2.0f, D[2], ez);
-  const float s = fmaf(ex, fx, fmaf(ey, fy, ez * fz));
+// acc += w K for one far quadrature-point pair in FP32 (without e^{ikR} and 1/(4 pi)),
+// given s = r^2 - R^2 and, for the double-layer kernels, dn = (x - y).n_y (DL) or
+// -(x - y).n_x (ADL). PH: phase of k (r - R): 0 range-reduced, 1 or 2 small tiers.
+template <int OP, bool HELM, int PH>
+__device__ __forceinline__ void far_accumulate_f32(float s, float dn, float R2, float R, float k, float w,
+                                                   cplxf& acc) {
   const float r2 = R2 + s;
   const float rinv = __builtin_amdgcn_rsqf(r2);
-  cplxf g;
-  float kr = 0.0f, sn = 0.0f, cs = 1.0f;
   if constexpr (HELM) {
     const float r = r2 * rinv;
-    const float delta = s * __builtin_amdgcn_rcpf(r + R);
-    if constexpr (WIDE)
-      sincos_f32_wide(k * delta, &sn, &cs);
+    const float kdelta = k * (s * __builtin_amdgcn_rcpf(r + R));  // k (r - R) without cancellation
+    float sn, cs;
+    if constexpr (PH == 0)
+      sincos_f32_wide(kdelta, &sn, &cs);
     else
-      sincos_small_f32(k * delta, &sn, &cs);
-    g = {cs * rinv, sn * rinv};
-    if constexpr (OP == 1 || OP == 2) kr = k * r;
+      sincos_small_f32<PH>(kdelta, &sn, &cs);
+    if constexpr (OP == 0 || OP == 3) {
+      const float rw = rinv * w;
+      acc.re = fmaf(cs, rw, acc.re);
+      acc.im = fmaf(sn, rw, acc.im);
+    } else {
+      // e^{ik(r-R)} (1 - ikr) dn / r^3
+      const float kr = k * r;
+      const float f = dn * (rinv * rinv) * (rinv * w);
+      acc.re = fmaf(fmaf(kr, sn, cs), f, acc.re);
+      acc.im = fmaf(fmaf(-kr, cs, sn), f, acc.im);
+    }
   } else {
-    g = {rinv, 0.0f};
-  }
-  if constexpr (OP == 0 || OP == 3) {
-    return g;
-  } else {
-    const float dx = D[0] + ex, dy = D[1] + ey, dz = D[2] + ez;
-    float dn;
-    if constexpr (OP == 1)
-      dn = fmaf(dx, ny[0], fmaf(dy, ny[1], dz * ny[2]));
+    if constexpr (OP == 0 || OP == 3)
+      acc.re = fmaf(rinv, w, acc.re);
     else
-      dn = -fmaf(dx, nx[0], fmaf(dy, nx[1], dz * nx[2]));
-    const float f = dn * rinv * rinv * rinv;
-    if constexpr (HELM)
-      return {fmaf(kr, sn, cs) * f, fmaf(-kr, cs, sn) * f};
-    else
-      return {f, 0.0f};
+      acc.re = fmaf(dn * (rinv * rinv), rinv * w, acc.re);
   }
 }
 
 // Mixed-precision P0 x P0 tiles: same tiling, symmetry and LDS mirror as
-// p0_tile; near and adjacent pairs are written as zero.
-template <int Q, int OP, bool HELM, bool WIDE, int CH = kChunk, typename Args>
+// p0_tile; near and adjacent pairs are written as zero. With offsets o_p, o_q of the
+// points from their cell centroids and D = c_test - c_trial,
+//   s_pq = r^2 - R^2 = a_p - b_q - 2 o_p.o_q,  a_p = o_p.(2D + o_p),  b_q = o_q.(2D - o_q),
+// so a cell pair costs 2Q short dot products and a point pair one 3-term dot product.
+// The double-layer normal products split the same way: (x - y).n = (D + o_p).n - o_q.n.
+template <int Q, int OP, bool HELM, int PH, int CH = kChunk, typename Args>
 __device__ __forceinline__ void p0_tile_mixed(Args& A, const int2 tile, int symmetric,
                                               double2 (&lds)[4][CH][kTile]) {
+  constexpr bool kDN = OP == 1 || OP == 2;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int i = tile.x * kTile + lane;
@@ -145,6 +161,7 @@ __device__ __forceinline__ void p0_tile_mixed(Args& A, const int2 tile, int symm
   const auto trial_vid = constant_view(A.trial_vid);
   float kf;
   asm volatile("v_mov_b32 %0, %1" : "=v"(kf) : "s"((float)A.k));
+  static_assert(CH == 4 || CH == 8 || CH == 16, "chunk must divide 16 and 64");
   for (int c = 0; c < 16 / CH; ++c) {
     const int jbase = tile.y * kTile + wave * 16 + c * CH;
 #pragma unroll 1
@@ -178,19 +195,38 @@ __device__ __forceinline__ void p0_tile_mixed(Args& A, const int2 tile, int symm
       if (far) {
         const double R = R2 * rsqrt_nr(R2);
         const float D[3] = {(float)Dx, (float)Dy, (float)Dz};
+        const float D2[3] = {2.0f * D[0], 2.0f * D[1], 2.0f * D[2]};
         const float R2f = (float)R2, Rf = (float)R;
+        const float* nrm = OP == 1 ? ny : nx;
+        const float sgn = OP == 2 ? -1.0f : 1.0f;
+        // D.n in FP64: (x - y).n ~ r^2 / 2 on smooth surfaces, well below |D| |n|
+        double Dn64 = 0.0;
+        if constexpr (OP == 1) Dn64 = Dx * sg[9] + Dy * sg[10] + Dz * sg[11];
+        if constexpr (OP == 2) Dn64 = -(Dx * tgeo[9] + Dy * tgeo[10] + Dz * tgeo[11]);
+        const float Dn = (float)Dn64;
+        float bq[Q], nq[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+          const float ox = sp[4 * q], oy = sp[4 * q + 1], oz = sp[4 * q + 2];
+          bq[q] = fmaf(D2[0] - ox, ox, fmaf(D2[1] - oy, oy, (D2[2] - oz) * oz));
+          if constexpr (kDN) nq[q] = sgn * fmaf(ox, nrm[0], fmaf(oy, nrm[1], oz * nrm[2]));
+        }
 #pragma unroll
         for (int p = 0; p < Q; ++p) {
-          cplxf ap = {0.0f, 0.0f};
+          const float ap = fmaf(D2[0] + op[p][0], op[p][0],
+                                fmaf(D2[1] + op[p][1], op[p][1], (D2[2] + op[p][2]) * op[p][2]));
+          const float mx = -2.0f * op[p][0], my = -2.0f * op[p][1], mz = -2.0f * op[p][2];
+          float dnp = 0.0f;
+          if constexpr (kDN)
+            dnp = fmaf(sgn * op[p][0], nrm[0], fmaf(sgn * op[p][1], nrm[1], fmaf(sgn * op[p][2], nrm[2], Dn)));
+          cplxf apq = {0.0f, 0.0f};
 #pragma unroll
           for (int q = 0; q < Q; ++q) {
-            const cplxf kv = kernel_scalar_f32<OP, HELM, WIDE>(op[p][0] - sp[4 * q], op[p][1] - sp[4 * q + 1],
-                                                         op[p][2] - sp[4 * q + 2], D, R2f, Rf, kf, nx, ny);
-            ap.re = fmaf(kv.re, sp[4 * q + 3], ap.re);
-            ap.im = fmaf(kv.im, sp[4 * q + 3], ap.im);
+            const float s = fmaf(mx, sp[4 * q], fmaf(my, sp[4 * q + 1], fmaf(mz, sp[4 * q + 2], ap - bq[q])));
+            far_accumulate_f32<OP, HELM, PH>(s, kDN ? dnp - nq[q] : 0.0f, R2f, Rf, kf, sp[4 * q + 3], apq);
           }
-          acc.re = fma((double)ap.re, wp[p], acc.re);
-          acc.im = fma((double)ap.im, wp[p], acc.im);
+          acc.re = fma((double)apq.re, wp[p], acc.re);
+          acc.im = fma((double)apq.im, wp[p], acc.im);
         }
         if constexpr (HELM) {
           cplx e;
@@ -223,11 +259,11 @@ __device__ __forceinline__ void p0_tile_mixed(Args& A, const int2 tile, int symm
   }
 }
 
-template <int Q, int OP, bool HELM, bool WIDE>
+template <int Q, int OP, bool HELM, int PH>
 __global__ void __launch_bounds__(256) regular_p0_tiled_mixed(const RegularArgs A, const int2* __restrict__ tiles,
                                                               int symmetric) {
   __shared__ double2 lds[4][kChunk][kTile];
-  p0_tile_mixed<Q, OP, HELM, WIDE>(A, tiles[blockIdx.x], symmetric, lds);
+  p0_tile_mixed<Q, OP, HELM, PH>(A, tiles[blockIdx.x], symmetric, lds);
 }
 
 // FP64 evaluation of a list of near (non-adjacent) P0 x P0 cell pairs, one
@@ -257,9 +293,9 @@ __device__ __forceinline__ void near_pair(Args& A, const int2 pr) {
   if constexpr (CENTRED) {
     double kR;
     const cplx e = pair_phase(tg + kGeoCentroid, sg + kGeoCentroid, A.k, &kR);
-    acc = cmul(p0_pair<Q, OP, HELM, true>(xp, wp, nx, sp, ny, A.k, kR), e);
+    acc = cmul(p0_pair<Q, OP, HELM, CENTRED>(xp, wp, nx, sp, ny, A.k, kR), e);
   } else {
-    acc = p0_pair<Q, OP, HELM, false>(xp, wp, nx, sp, ny, A.k, 0.0);
+    acc = p0_pair<Q, OP, HELM, 0>(xp, wp, nx, sp, ny, A.k, 0.0);
   }
   if constexpr (OP == 3) {
     const double nn = HELM ? -A.k * A.k * (nx[0] * ny[0] + nx[1] * ny[1] + nx[2] * ny[2]) : 0.0;
@@ -279,12 +315,12 @@ __global__ void __launch_bounds__(256) near_pair_kernel(const RegularArgs A, con
 
 // ---------------------------------------------------------------------------
 // Batched mixed precision: persistent launches over the tiles of all items (FP32
-// far field; one launch for centred items, one with the WIDE phase reduction for
-// the others), then the near-pair lists of all items concatenated (item i owns
-// near pairs [first[i], first[i + 1])), one thread per pair.
+// far field; one launch per phase tier PH of the items), then the near-pair lists of
+// all items concatenated (item i owns near pairs [first[i], first[i + 1])), one
+// thread per pair.
 // ---------------------------------------------------------------------------
-template <int Q, int OP, bool HELM, bool WIDE, int CH>
-__global__ void __launch_bounds__(256) batched_p0_mixed_persistent(const RegularArgs* __restrict__ items,
+template <int Q, int OP, bool HELM, int PH, int CH>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) batched_p0_mixed_persistent(const RegularArgs* __restrict__ items,
                                                                    const BatchTile* __restrict__ tiles, int ntiles,
                                                                    int* counter, int symmetric) {
   __shared__ double2 lds[4][CH][kTile];
@@ -296,7 +332,7 @@ __global__ void __launch_bounds__(256) batched_p0_mixed_persistent(const Regular
     __syncthreads();
     if (t >= ntiles) return;
     const BatchTile bt = load_uniform(tiles + t);
-    p0_tile_mixed<Q, OP, HELM, WIDE, CH>(constant_view(items)[bt.item], make_int2(bt.a, bt.b), symmetric, lds);
+    p0_tile_mixed<Q, OP, HELM, PH, CH>(constant_view(items)[bt.item], make_int2(bt.a, bt.b), symmetric, lds);
   }
 }
 
