@@ -20,6 +20,11 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (and matrix) peak
 # 1/r: 1, kr: 1, sin, cos: 2, e^{ikr}/r: 2, weight: 1, complex accumulate: 4) and per pair epilogue
 FLOP_PER_QPP = 20
 FLOP_PER_PAIR_EPILOGUE = 4
+# HBM bytes per launch of the FP64 tile kernel at the default workload (level 5, k = 5),
+# from rocprofv3 --pmc FETCH_SIZE (x 2.00) and WRITE_SIZE (x 1.00), the factors being what
+# the counters read for a 1 GiB device copy (bench.py --copy-calibration); see
+# profiles/regular_p0_tiled_helmholtz_sl_p0_l5_k5.md
+TILE_KERNEL_TRAFFIC_L5_K5 = 1.092e9 + 6.711e9
 
 
 def adjacent_ordered_pairs(cells, nv):
@@ -152,11 +157,79 @@ def copy_calibration():
     print(json.dumps({"copy_calibration_bytes_per_dispatch": n * 8, "dispatches": 3}))
 
 
+def config_suite(steps, warmup):
+    """Every dense configuration of the workload set, timed on the GPU (HIP events around the
+    kernels, one launch sequence per step, output in HBM) with 32 sample rows checked against
+    the FP64 oracle: one JSON line per configuration."""
+    import numpy as np
+    import torch
+
+    import oracle_lib as O
+    from bempp_amd.assembly.boundary import (BoundaryAssemblerOptions, OperatorType, Precision,
+                                             create_helmholtz_assembler, create_laplace_assembler)
+    from bempp_amd.element import Continuity, Family, ReferenceCellType, create_family
+    from bempp_amd.function_space import function_space
+    from bempp_amd.shapes import icosphere
+
+    ops = {"single_layer": OperatorType.SingleLayer, "double_layer": OperatorType.DoubleLayer,
+           "adjoint_double_layer": OperatorType.AdjointDoubleLayer, "hypersingular": OperatorType.Hypersingular}
+    cases = [("laplace", "single_layer", "dp0", 3, 0.0, "fp64", np.float64),
+             ("laplace", "double_layer", "p1", 5, 0.0, "fp64", np.float64),
+             ("laplace", "hypersingular", "p1", 5, 0.0, "fp64", np.float64),
+             ("helmholtz", "single_layer", "dp0", 5, 5.0, "fp64", np.complex128),
+             ("helmholtz", "single_layer", "dp0", 5, 5.0, "mixed", np.complex64),
+             ("helmholtz", "double_layer", "p1", 5, 5.0, "fp64", np.complex128),
+             ("helmholtz", "adjoint_double_layer", "p1", 5, 5.0, "fp64", np.complex128),
+             ("helmholtz", "hypersingular", "p1", 5, 5.0, "fp64", np.complex128),
+             ("laplace", "single_layer", "dp0", 6, 0.0, "fp64", np.float64)]
+    grids = {}
+    for kind, op, sp, level, k, prec, dt in cases:
+        g = grids.setdefault(level, icosphere(level))
+        space = function_space(g, create_family(Family.Lagrange, 1 if sp == "p1" else 0,
+                                                Continuity.Standard if sp == "p1" else Continuity.Discontinuous))
+        o = BoundaryAssemblerOptions()
+        o.set_regular_quadrature_degree(ReferenceCellType.Triangle, 6)
+        o.set_singular_quadrature_degree(ReferenceCellType.Triangle, ReferenceCellType.Triangle, 4)
+        if prec == "mixed":
+            o.set_precision(Precision.Mixed)
+        a = (create_laplace_assembler(ops[op], o, dtype=dt) if kind == "laplace"
+             else create_helmholtz_assembler(k, ops[op], o, dtype=dt))
+        a.set_profiling(True)
+        n = space.global_size
+        tdt = {np.float64: torch.float64, np.complex128: torch.complex128, np.complex64: torch.complex64}[dt]
+        out = torch.empty((n, n), dtype=tdt, device="cuda").t()
+        for _ in range(warmup):
+            a.assemble_into_device(space, space, out)
+        torch.cuda.synchronize()
+        times = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            a.assemble_into_device(space, space, out)
+            times.append(a.kernel_times())
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        rows = min(32, n)
+        err = None  # the oracle restricts rows for P0 test spaces only; P1 parity is in the GPU tests
+        if sp == "dp0":
+            ref = O.assemble(kind, op, g.points, g.cells, k=k, rows=(0, rows))
+            got = out[:rows].cpu().numpy().astype(ref.dtype)
+            err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref)) if np.linalg.norm(ref) > 0 else 0.0
+        line = {"config": f"{kind}_{op}_{sp}_icosphere_l{level}" + (f"_k{k:g}" if kind == "helmholtz" else ""),
+                "precision": prec, "output": np.dtype(dt).name, "dofs": n, "entries": n * n, "ms_per_assembly": ms,
+                "entries_per_s": n * n / (ms * 1e-3),
+                "regular_kernels_ms": float(np.median([t[0] for t in times])),
+                "adjacent_kernel_ms": float(np.median([t[1] for t in times])),
+                "sample_rows": rows, "sample_rel_err_vs_oracle": err}
+        print(json.dumps(line), flush=True)
+        del out
+        torch.cuda.empty_cache()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=5)  # the first launches after idle run at lower clocks
     ap.add_argument("--level", type=int, default=5)
     ap.add_argument("--k", type=float, default=5.0)
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
@@ -165,11 +238,16 @@ def main():
     ap.add_argument("--batch-seed", type=int, default=0)
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--configs", action="store_true",
+                    help="time every dense configuration (one JSON line each) instead of the metric")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB copy used to calibrate the HBM byte counters")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.configs:
+        config_suite(max(1, min(args.steps, 5)), max(1, min(args.warmup, 3)))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine assembles on one GPU; multi-GPU runs are not part of this build "
@@ -224,28 +302,41 @@ def main():
     got = out[:rows].cpu().numpy()
     sample_err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
 
-    # mixed precision: FP32 far pairs, FP64 near/adjacent pairs, same output dtype
+    # mixed precision: FP32 far pairs, FP64 near/adjacent pairs and accumulation; the
+    # complex64-mixed form (k = 5 is exact in FP32) and, for comparison, complex128 output
     from bempp_amd.assembly.boundary import Precision
 
-    mopts = BoundaryAssemblerOptions()
-    mopts.set_regular_quadrature_degree(ReferenceCellType.Triangle, 6)
-    mopts.set_singular_quadrature_degree(ReferenceCellType.Triangle, ReferenceCellType.Triangle, 4)
-    mopts.set_precision(Precision.Mixed)
-    masm = create_helmholtz_assembler(args.k, OperatorType.SingleLayer, mopts)
-    masm.set_profiling(True)
-    for _ in range(args.warmup):
-        masm.assemble_into_device(space, space, out, stream=stream)
-    torch.cuda.synchronize()
-    mreg = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        masm.assemble_into_device(space, space, out, stream=stream)
-        mreg.append(masm.kernel_times()[0])
-    torch.cuda.synchronize()
-    mixed_ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    mixed_err = float(np.linalg.norm(out[:rows].cpu().numpy() - ref) / np.linalg.norm(ref))
-    print(f"[bench] mixed: {mixed_ms:.3f} ms/step, tile kernel {np.mean(mreg):.3f} ms, sample rel err {mixed_err:.2e}",
-          file=sys.stderr, flush=True)
+    mixed = {}
+    for mdt, tdt in ((np.complex64, torch.complex64), (np.complex128, torch.complex128)):
+        mopts = BoundaryAssemblerOptions()
+        mopts.set_regular_quadrature_degree(ReferenceCellType.Triangle, 6)
+        mopts.set_singular_quadrature_degree(ReferenceCellType.Triangle, ReferenceCellType.Triangle, 4)
+        mopts.set_precision(Precision.Mixed)
+        masm = create_helmholtz_assembler(args.k, OperatorType.SingleLayer, mopts, dtype=mdt)
+        masm.set_profiling(True)
+        mout = out if mdt == np.complex128 else torch.empty((n, n), dtype=tdt, device="cuda").t()
+        for _ in range(args.warmup):
+            masm.assemble_into_device(space, space, mout, stream=stream)
+        torch.cuda.synchronize()
+        mreg, msing = [], []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            masm.assemble_into_device(space, space, mout, stream=stream)
+            r, s_ = masm.kernel_times()
+            mreg.append(r)
+            msing.append(s_)
+        torch.cuda.synchronize()
+        mms = (time.perf_counter() - t0) * 1e3 / args.steps
+        merr = float(np.linalg.norm(mout[:rows].cpu().numpy().astype(np.complex128) - ref) / np.linalg.norm(ref))
+        key = "complex64" if mdt == np.complex64 else "complex128"
+        mixed[key] = {"value": entries / (mms * 1e-3), "ms_per_step": mms, "speedup_vs_fp64": ms_per_step / mms,
+                      "sample_rel_err_vs_oracle": merr, "regular_kernels_ms": float(np.mean(mreg)),
+                      "adjacent_kernel_ms": float(np.mean(msing))}
+        print(f"[bench] mixed {key}: {mms:.3f} ms/step, far+near kernels {np.mean(mreg):.3f} ms, adjacent "
+              f"{np.mean(msing):.3f} ms, sample rel err {merr:.2e}", file=sys.stderr, flush=True)
+        if mout is not out:
+            del mout
+            torch.cuda.empty_cache()
 
     n_adj = adjacent_ordered_pairs(grid.cells, len(grid.points))
     n_reg = entries - n_adj
@@ -259,11 +350,11 @@ def main():
         batched = {"workload": f"{args.batch_items} Helmholtz SL P0 assemblies: icospheres of level uniform in 2..5 "
                                f"scaled to ellipsoids (semi-axes uniform in [0.5, 2]), k uniform in [1, 10], "
                                f"numpy seed {args.batch_seed}"}
-        for mixed in (False, True):
-            e, res = batch_leg(args.batch_seed, args.batch_items, args.batch_steps, mixed)
+        for bmixed in (False, True):
+            e, res = batch_leg(args.batch_seed, args.batch_items, args.batch_steps, bmixed)
             batched["entries"] = e
-            batched["mixed_complex64" if mixed else "fp64_complex128"] = res
-            print(f"[bench] batch {'mixed c64' if mixed else 'fp64 c128'}: {res['ms_per_step']:.2f} ms, per-item "
+            batched["mixed_complex64" if bmixed else "fp64_complex128"] = res
+            print(f"[bench] batch {'mixed c64' if bmixed else 'fp64 c128'}: {res['ms_per_step']:.2f} ms, per-item "
                   f"launches {res['per_item_launches_ms']:.2f} ms, sample err {res['sample_rel_err_vs_oracle']:.2e}",
                   file=sys.stderr, flush=True)
 
@@ -284,13 +375,13 @@ def main():
                    "entries": entries, "regular_points": 6, "duffy_gauss_points": 4, "precision": "fp64",
                    "output": "device-resident complex128 column-major"},
         "sample_rel_err_vs_oracle": sample_err,
-        "mixed": {"value": entries / (mixed_ms * 1e-3), "ms_per_step": mixed_ms, "speedup_vs_fp64": ms_per_step / mixed_ms,
-                  "sample_rel_err_vs_oracle": mixed_err, "regular_kernels_ms": float(np.mean(mreg)),
-                  "scheme": "FP32 far pairs (R >= 3 (rad_i + rad_j)) on centroid offsets, FP64 e^{ikR}, "
-                            "near and adjacent pairs FP64"},
+        "mixed": dict(mixed["complex64"], output="complex64", complex128_output=mixed["complex128"],
+                      scheme="FP32 far pairs (R >= 3 (rad_i + rad_j)) on centroid offsets with FP64 cell-pair "
+                             "phase reduction; near and adjacent pairs and all accumulation FP64"),
         "kernels_ms": {"regular": float(np.mean(reg_ms)), "adjacent": float(np.mean(sing_ms))},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": TILE_KERNEL_TRAFFIC_L5_K5 if (args.level, args.k) == (5, 5.0) else None,
                      "basis": f"{n_reg} regular pairs x (36 x {FLOP_PER_QPP} + {FLOP_PER_PAIR_EPILOGUE}) nominal "
                               "flop per launch / mean regular-kernel HIP-event time"},
         "cpu_baseline": cpu,

@@ -243,3 +243,19 @@ def test_mixed_precision_wide_phase(op):
     print(f"[mixed-wide]   rel Frobenius err {err:.3e}, max entry err {emax:.3e}", flush=True)
     assert err <= 1e-6 and emax <= 1e-5
     assert err > 1e-12
+
+
+@pytest.mark.parametrize("op", ["single_layer", "double_layer"])
+def test_mixed_precision_complex64_output(op):
+    """The complex64-mixed form: FP32 far field, FP64 near/adjacent pairs and accumulation,
+    complex64 storage; k = 2.5 is exact in FP32. Level-4 icosphere, <= 1e-6 against the
+    FP64 oracle."""
+    g = icosphere(4)
+    s = _space(g, "dp0")
+    a = create_helmholtz_assembler(2.5, OPS[op], _mixed_options(), dtype=np.complex64)
+    A = a.assemble(s, s)
+    assert A.dtype == np.complex64
+    R = O.assemble("helmholtz", op, g.points, g.cells, k=2.5)
+    err = _rel(A.astype(np.complex128), R)
+    print(f"[mixed-c64] {op}: rel Frobenius err {err:.3e}", flush=True)
+    assert 1e-12 < err <= 1e-6
