@@ -42,6 +42,11 @@ struct DeviceGridCache {
 struct DeviceSpaceCache {
   int device = -1;
   int* dofs = nullptr;  // [nc][nb] int32
+  // 64-cell blocks (tiled kernel with a P1 side), built on first use
+  int* blk_off = nullptr;   // [nblocks + 1]
+  int* blk_dofs = nullptr;  // unique DOFs per block, ascending
+  int* ldof = nullptr;      // [nc][nb] position in the block's list
+  int max_nd = -1;          // largest block DOF count
 };
 
 void free_device_grid_cache(DeviceGridCache* c) {
@@ -64,6 +69,9 @@ void free_device_space_cache(DeviceSpaceCache* c) {
   int cur = 0;
   if (hipGetDevice(&cur) == hipSuccess && hipSetDevice(c->device) == hipSuccess) {
     (void)hipFree(c->dofs);
+    if (c->blk_off) (void)hipFree(c->blk_off);
+    if (c->blk_dofs) (void)hipFree(c->blk_dofs);
+    if (c->ldof) (void)hipFree(c->ldof);
     (void)hipSetDevice(cur);
   }
   delete c;
@@ -319,6 +327,38 @@ const int* space_dofs(FunctionSpace& s, int device) {
   return c->dofs;
 }
 
+// Block DOF lists of a space for the tiled kernel with a P1 side: cells [64 t, 64 t + 64)
+// form block t (consecutive cells of a refined mesh are a compact patch).
+const DeviceSpaceCache* space_blocks(FunctionSpace& s, int device) {
+  space_dofs(s, device);
+  std::lock_guard<std::mutex> lk(s.cache_mutex);
+  DeviceSpaceCache* c = s.device_cache.at(device);
+  if (c->max_nd >= 0) return c;
+  const int64_t nc = s.grid->nc();
+  const int nb = s.nb;
+  const int64_t nblk = (nc + dev::kTile - 1) / dev::kTile;
+  std::vector<int> off(nblk + 1, 0), bd, ldof((size_t)nc * nb);
+  int max_nd = 0;
+  std::vector<int64_t> tmp;
+  for (int64_t t = 0; t < nblk; ++t) {
+    const int64_t c0 = t * dev::kTile, c1 = std::min(nc, c0 + dev::kTile);
+    tmp.assign(s.dofs.begin() + c0 * nb, s.dofs.begin() + c1 * nb);
+    std::sort(tmp.begin(), tmp.end());
+    tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+    for (int64_t cc = c0; cc < c1; ++cc)
+      for (int a = 0; a < nb; ++a)
+        ldof[cc * nb + a] = (int)(std::lower_bound(tmp.begin(), tmp.end(), s.dofs[cc * nb + a]) - tmp.begin());
+    for (int64_t d : tmp) bd.push_back((int)d);
+    off[t + 1] = (int)bd.size();
+    max_nd = std::max(max_nd, (int)tmp.size());
+  }
+  c->blk_off = upload(off);
+  c->blk_dofs = upload(bd);
+  c->ldof = upload(ldof);
+  c->max_nd = max_nd;
+  return c;
+}
+
 // Canonical Sauter-Schwab rules for one Gauss order n, per device (kept for the
 // life of the process: a few hundred KB).
 struct RuleTable {
@@ -391,6 +431,10 @@ int singular_points(const Options& o) {
   return (int)it->second;
 }
 
+// Dynamic LDS budget of the LDS-accumulating tile kernel (block DOF lists of a compact
+// 64-cell P1 patch: ~45 x 45 x 16 B = 32 KB).
+constexpr size_t kMaxTileLds = 64 * 1024;
+
 // Tile coordinates (tile row = test block, tile column = trial block) for the
 // P0 x P0 kernel: all tiles, or the upper triangle a <= b when symmetric.
 // Cached per device and shape for the life of the process.
@@ -413,11 +457,15 @@ const int2* tile_list(int device, int nta, int ntb, bool symmetric, int* count) 
 // ---- kernel dispatch: tiled P0 x P0 or general, by operator, kernel and phase mode ----
 // CENTRED: phase tier (0: full reduction, 1: |k (r - R)| <= 0.5, 2: <= 0.25; tier 2
 // only for the P0 x P0 tile kernel, the general kernel runs tier 1 there).
+// mode: 0 general kernel (atomics per cell pair), 1 P0 x P0 tiles, 2 tiles with LDS
+// accumulation (a P1 side; lds = dynamic LDS bytes).
 template <int OP, bool HELM, int CENTRED>
-void launch_q(bool tiled, int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& A, const int2* tiles, int sym) {
+void launch_q(int mode, int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& A, const int2* tiles, int sym,
+              size_t lds) {
   constexpr int GC = CENTRED > 1 ? 1 : CENTRED;
-#define BEMPP_LAUNCH(QQ)                                                                 \
-  if (tiled) dev::regular_p0_tiled<QQ, OP, HELM, CENTRED><<<grid, 256, 0, s>>>(A, tiles, sym); \
+#define BEMPP_LAUNCH(QQ)                                                                                \
+  if (mode == 1) dev::regular_p0_tiled<QQ, OP, HELM, CENTRED><<<grid, 256, 0, s>>>(A, tiles, sym);       \
+  else if (mode == 2) dev::regular_tiled_lds<QQ, OP, HELM, GC><<<grid, 256, lds, s>>>(A, tiles, sym);    \
   else dev::regular_kernel<QQ, OP, HELM, GC><<<grid, 256, 0, s>>>(A);
   switch (Q) {
     case 1: BEMPP_LAUNCH(1); break;
@@ -430,25 +478,25 @@ void launch_q(bool tiled, int Q, dim3 grid, hipStream_t s, const dev::RegularArg
 }
 
 template <bool HELM, int CENTRED>
-void launch_op(int op, bool tiled, int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& A, const int2* tiles,
-               int sym) {
+void launch_op(int op, int mode, int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& A, const int2* tiles,
+               int sym, size_t lds) {
   switch (op) {
-    case kSL: launch_q<kSL, HELM, CENTRED>(tiled, Q, grid, s, A, tiles, sym); break;
-    case kDL: launch_q<kDL, HELM, CENTRED>(tiled, Q, grid, s, A, tiles, sym); break;
-    case kADL: launch_q<kADL, HELM, CENTRED>(tiled, Q, grid, s, A, tiles, sym); break;
-    case kHS: launch_q<kHS, HELM, CENTRED>(tiled, Q, grid, s, A, tiles, sym); break;
+    case kSL: launch_q<kSL, HELM, CENTRED>(mode, Q, grid, s, A, tiles, sym, lds); break;
+    case kDL: launch_q<kDL, HELM, CENTRED>(mode, Q, grid, s, A, tiles, sym, lds); break;
+    case kADL: launch_q<kADL, HELM, CENTRED>(mode, Q, grid, s, A, tiles, sym, lds); break;
+    case kHS: launch_q<kHS, HELM, CENTRED>(mode, Q, grid, s, A, tiles, sym, lds); break;
     default: throw Error("operator not supported by the dense assembler");
   }
 }
 
-// Regular pairs: `tiled` selects the P0 x P0 tile kernel (grid = tile count),
-// otherwise the general kernel (grid = test blocks x trial chunks).
-void launch_regular(int op, bool helm, int tier, bool tiled, int Q, dim3 grid, hipStream_t s,
-                    const dev::RegularArgs& A, const int2* tiles = nullptr, int sym = 0) {
-  if (!helm) launch_op<false, 0>(op, tiled, Q, grid, s, A, tiles, sym);
-  else if (tier == 2 && tiled) launch_op<true, 2>(op, tiled, Q, grid, s, A, tiles, sym);
-  else if (tier >= 1) launch_op<true, 1>(op, tiled, Q, grid, s, A, tiles, sym);
-  else launch_op<true, 0>(op, tiled, Q, grid, s, A, tiles, sym);
+// Regular pairs: mode 1 the P0 x P0 tile kernel, mode 2 the LDS-accumulating tile kernel
+// (grid = tile count), mode 0 the general kernel (grid = test blocks x trial chunks).
+void launch_regular(int op, bool helm, int tier, int mode, int Q, dim3 grid, hipStream_t s,
+                    const dev::RegularArgs& A, const int2* tiles = nullptr, int sym = 0, size_t lds = 0) {
+  if (!helm) launch_op<false, 0>(op, mode, Q, grid, s, A, tiles, sym, lds);
+  else if (tier == 2 && mode == 1) launch_op<true, 2>(op, mode, Q, grid, s, A, tiles, sym, lds);
+  else if (tier >= 1) launch_op<true, 1>(op, mode, Q, grid, s, A, tiles, sym, lds);
+  else launch_op<true, 0>(op, mode, Q, grid, s, A, tiles, sym, lds);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -699,11 +747,32 @@ void assemble_device(const Assembler& a, const FunctionSpace& trial, const Funct
         launch_mixed(a.op, a.kind == kHelmholtz, P.tier, P.Q, dim3(ntiles), P.stream, R, tiles, sym ? 1 : 0,
                      P.near, P.nnear);
       } else {
-        launch_regular(a.op, a.kind == kHelmholtz, P.tier, true, P.Q, dim3(ntiles), P.stream, R, tiles,
+        launch_regular(a.op, a.kind == kHelmholtz, P.tier, 1, P.Q, dim3(ntiles), P.stream, R, tiles,
                        sym ? 1 : 0);
       }
     } else {
-      launch_regular(a.op, a.kind == kHelmholtz, P.tier, false, P.Q, grid, P.stream, R);
+      // a P1 side: tiles with LDS accumulation when the blocks' DOF lists fit in LDS
+      const DeviceSpaceCache* tb = space_blocks(const_cast<FunctionSpace&>(test), P.device);
+      const DeviceSpaceCache* sb = space_blocks(const_cast<FunctionSpace&>(trial), P.device);
+      const size_t lds = (size_t)tb->max_nd * (size_t)sb->max_nd * sizeof(double2);
+      if (lds <= kMaxTileLds) {
+        R.test_blk_off = tb->blk_off;
+        R.test_blk_dofs = tb->blk_dofs;
+        R.test_ldof = tb->ldof;
+        R.trial_blk_off = sb->blk_off;
+        R.trial_blk_dofs = sb->blk_dofs;
+        R.trial_ldof = sb->ldof;
+        // symmetric operators on one space (the DOF numbering is fixed by grid and family)
+        const bool sym = (a.op == kSL || a.op == kHS) && P.same_grid && test.nb == trial.nb &&
+                         test.degree == trial.degree && test.continuous == trial.continuous;
+        const int nta = (R.n_test + dev::kTile - 1) / dev::kTile, ntb = (R.n_trial + dev::kTile - 1) / dev::kTile;
+        int ntiles = 0;
+        const int2* tiles = tile_list(P.device, nta, ntb, sym, &ntiles);
+        launch_regular(a.op, a.kind == kHelmholtz, P.tier, 2, P.Q, dim3(ntiles), P.stream, R, tiles, sym ? 1 : 0,
+                       lds);
+      } else {
+        launch_regular(a.op, a.kind == kHelmholtz, P.tier, 0, P.Q, grid, P.stream, R);
+      }
     }
     if (a.profiling) {
       HIP_CHECK(hipEventRecord((hipEvent_t)am.events[1], P.stream));

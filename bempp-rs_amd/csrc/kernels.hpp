@@ -29,6 +29,8 @@ constexpr int kGeoStride = 32;
 constexpr int kGeoCentroid = 24;
 constexpr int kGeoRadius = 27;
 constexpr int kMaxQ = 7;
+constexpr int kTile = 64;   // cells per tile side (lanes of a wavefront)
+constexpr int kChunk = 8;   // trial cells per chunk of the P0 tile kernel
 
 enum OutType { kOutF64 = 0, kOutC128 = 1, kOutF32 = 2, kOutC64 = 3 };
 enum Mode { kStore = 0, kAccumulate = 1, kAtomic = 2, kCoo = 3 };
@@ -44,6 +46,15 @@ struct RegularArgs {
   const int* trial_vid;     // [n_trial][4]
   const int* test_dofs;     // [n_test][nbt]
   const int* trial_dofs;    // [n_trial][nbs]
+  // DOFs of 64-cell blocks (tiled kernel with a P1 side): unique DOFs of block t are
+  // blk_dofs[blk_off[t] .. blk_off[t+1]) in ascending order; ldof[c][a] is the position
+  // of cell c's a-th DOF in its block's list
+  const int* test_blk_off;
+  const int* test_blk_dofs;
+  const int* test_ldof;     // [n_test][nbt]
+  const int* trial_blk_off;
+  const int* trial_blk_dofs;
+  const int* trial_ldof;    // [n_trial][nbs]
   int n_test, n_trial;
   int nbt, nbs;
   int same_grid;
@@ -190,15 +201,15 @@ __device__ __forceinline__ void store_value(void* out, long long idx, cplx v, in
 // Epilogue shared by both loops: M (nbt x nbs local sums without 1/(4pi)) ->
 // local matrix entries. tgeo/sgeo: geometry records; lt/ls: local basis index
 // of each row/column of M.
-template <int OP, bool HELM>
+template <int OP, bool HELM, typename TG, typename SG>
 __device__ __forceinline__ cplx local_entry(const cplx M[3][3], int a, int b, cplx msum, int nbt, int nbs,
-                                            const double* tgeo, const double* sgeo, int la, int lb, double k) {
+                                            TG tgeo, SG sgeo, int la, int lb, double k) {
   cplx v = M[a][b];
   if constexpr (OP == 3) {
     cplx out = {0.0, 0.0};
     if (nbt == 3 && nbs == 3) {
-      const double* ct = tgeo + 13 + 3 * la;
-      const double* cs = sgeo + 13 + 3 * lb;
+      const auto ct = tgeo + 13 + 3 * la;
+      const auto cs = sgeo + 13 + 3 * lb;
       const double cc = (ct[0] * cs[0] + ct[1] * cs[1] + ct[2] * cs[2]) / (tgeo[12] * sgeo[12]);
       out = {cc * msum.re, cc * msum.im};
     }
@@ -249,6 +260,59 @@ __device__ __forceinline__ cplx cmul(cplx a, cplx b) {
   return {fma(a.re, b.re, -a.im * b.im), fma(a.re, b.im, a.im * b.re)};
 }
 
+// Local block M_ab (without 1/(4 pi)) of one regular cell pair with a P1 side:
+// M_ab = sum_p sum_q K_pq (phi_a(p) w_p J_x)(psi_b(q) w_q J_y), summed over q first
+// into T_b(p); msum = sum_ab M_ab (for the hypersingular curl-curl term).
+template <int Q, int OP, bool HELM, int CENTRED, typename Args, typename SG>
+__device__ __forceinline__ void p1_local(Args& A, const double (&xp)[Q][3], const double (&wp)[Q],
+                                         const double (&nx)[3], const double (&ct)[3], const double (&sp)[Q * 4],
+                                         SG sgeo, cplx (&M)[3][3], cplx& msum) {
+  const double ny[3] = {sgeo[9], sgeo[10], sgeo[11]};
+  double kR = 0.0;
+  cplx e = {1.0, 0.0};
+  if constexpr (CENTRED) {
+    const double cs[3] = {sgeo[kGeoCentroid], sgeo[kGeoCentroid + 1], sgeo[kGeoCentroid + 2]};
+    e = pair_phase(ct, cs, A.k, &kR);
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) M[a][b] = {0.0, 0.0};
+#pragma unroll
+  for (int p = 0; p < Q; ++p) {
+    cplx T[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const cplx kv = kernel_scalar<OP, HELM, CENTRED>(xp[p][0] - sp[4 * q], xp[p][1] - sp[4 * q + 1],
+                                                       xp[p][2] - sp[4 * q + 2], A.k, nx, ny, kR);
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        const double f = (A.nbs == 3 ? A.phi[q][b] : (b == 0 ? 1.0 : 0.0)) * sp[4 * q + 3];
+        T[b].re = fma(kv.re, f, T[b].re);
+        T[b].im = fma(kv.im, f, T[b].im);
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double f = (A.nbt == 3 ? A.phi[p][a] : (a == 0 ? 1.0 : 0.0)) * wp[p];
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        M[a][b].re = fma(f, T[b].re, M[a][b].re);
+        M[a][b].im = fma(f, T[b].im, M[a][b].im);
+      }
+    }
+  }
+  msum = {0.0, 0.0};
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      if constexpr (CENTRED) M[a][b] = cmul(M[a][b], e);
+      msum.re += M[a][b].re;
+      msum.im += M[a][b].im;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Regular pairs with a P1 space on either side. One thread per test cell; the
 // block's trial cells are walked in order with wave-uniform (scalar) loads of
@@ -283,63 +347,108 @@ __global__ void __launch_bounds__(256) regular_kernel(const RegularArgs A) {
 #pragma unroll
     for (int m = 0; m < Q * 4; ++m) sp[m] = spp[m];
     const auto sg = trial_geo + (size_t)j * kGeoStride;
-    double sgeo[kGeoStride];
-#pragma unroll
-    for (int m = 0; m < kGeoStride; ++m) sgeo[m] = sg[m];
     const int sv0 = trial_vid[4 * j], sv1 = trial_vid[4 * j + 1], sv2 = trial_vid[4 * j + 2];
     const bool adjacent = A.same_grid && (tv0 == sv0 || tv0 == sv1 || tv0 == sv2 || tv1 == sv0 || tv1 == sv1 ||
                                           tv1 == sv2 || tv2 == sv0 || tv2 == sv1 || tv2 == sv2);
     if (adjacent || !i_ok) continue;
-    const double ny[3] = {sgeo[9], sgeo[10], sgeo[11]};
-    double kR = 0.0;
-    cplx e = {1.0, 0.0};
-    if constexpr (CENTRED) e = pair_phase(ct, sgeo + kGeoCentroid, A.k, &kR);
-    cplx M[3][3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-      for (int b = 0; b < 3; ++b) M[a][b] = {0.0, 0.0};
-#pragma unroll
-    for (int p = 0; p < Q; ++p) {
-      cplx T[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
-#pragma unroll
-      for (int q = 0; q < Q; ++q) {
-        const cplx kv = kernel_scalar<OP, HELM, CENTRED>(xp[p][0] - sp[4 * q], xp[p][1] - sp[4 * q + 1],
-                                                         xp[p][2] - sp[4 * q + 2], A.k, nx, ny, kR);
-#pragma unroll
-        for (int b = 0; b < 3; ++b) {
-          const double f = (A.nbs == 3 ? A.phi[q][b] : (b == 0 ? 1.0 : 0.0)) * sp[4 * q + 3];
-          T[b].re = fma(kv.re, f, T[b].re);
-          T[b].im = fma(kv.im, f, T[b].im);
-        }
-      }
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const double f = (A.nbt == 3 ? A.phi[p][a] : (a == 0 ? 1.0 : 0.0)) * wp[p];
-#pragma unroll
-        for (int b = 0; b < 3; ++b) {
-          M[a][b].re = fma(f, T[b].re, M[a][b].re);
-          M[a][b].im = fma(f, T[b].im, M[a][b].im);
-        }
-      }
-    }
-    cplx msum = {0.0, 0.0};
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-      for (int b = 0; b < 3; ++b) {
-        if constexpr (CENTRED) M[a][b] = cmul(M[a][b], e);
-        msum.re += M[a][b].re;
-        msum.im += M[a][b].im;
-      }
+    cplx M[3][3], msum;
+    p1_local<Q, OP, HELM, CENTRED>(A, xp, wp, nx, ct, sp, sg, M, msum);
     for (int b = 0; b < A.nbs; ++b) {
       const long long col = A.trial_dofs[(size_t)j * A.nbs + b];
       for (int a = 0; a < A.nbt; ++a) {
         const long long row = A.test_dofs[(size_t)i * A.nbt + a];
-        const cplx v = local_entry<OP, HELM>(M, a, b, msum, A.nbt, A.nbs, tgeo, sgeo, a, b, A.k);
+        const cplx v = local_entry<OP, HELM>(M, a, b, msum, A.nbt, A.nbs, tgeo, sg, a, b, A.k);
         store_value(A.out, row + A.ld * col, v, A.out_type, kAtomic);
       }
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Regular pairs with a P1 side, tiled: one workgroup (4 wavefronts) per tile of 64 test
+// x 64 trial cells (lane = test cell, 16 trial cells per wavefront). The local blocks of
+// the tile's cell pairs are summed in LDS at (block-local test DOF, block-local trial DOF)
+// -- a compact 64-cell patch of a P1 space has ~45 vertices, so the tile touches ~45 x 45
+// entries instead of 64 x 64 x 9 -- and each non-zero LDS entry is then added to the
+// output with one global atomic. symmetric (single layer / hypersingular on one space):
+// tiles a <= b only; an off-diagonal tile also adds its entries at the transposed place.
+// Dynamic LDS: (max block DOFs)^2 double2.
+// ---------------------------------------------------------------------------
+template <int Q, int OP, bool HELM, int CENTRED>
+__global__ void __launch_bounds__(256) regular_tiled_lds(const RegularArgs A, const int2* __restrict__ tiles,
+                                                         int symmetric) {
+  extern __shared__ double2 acc[];
+  const int2 tile = tiles[blockIdx.x];
+  const int t0 = A.test_blk_off[tile.x], ndt = A.test_blk_off[tile.x + 1] - t0;
+  const int s0 = A.trial_blk_off[tile.y], nds = A.trial_blk_off[tile.y + 1] - s0;
+  const int nacc = ndt * nds;
+  for (int e = threadIdx.x; e < nacc; e += blockDim.x) acc[e] = make_double2(0.0, 0.0);
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i = tile.x * kTile + lane;
+  const bool i_ok = i < A.n_test;
+  const int il = i_ok ? i : A.n_test - 1;
+  double xp[Q][3], wp[Q];
+#pragma unroll
+  for (int p = 0; p < Q; ++p) {
+    const double* s = A.test_pts + ((size_t)il * Q + p) * 4;
+    xp[p][0] = s[0];
+    xp[p][1] = s[1];
+    xp[p][2] = s[2];
+    wp[p] = s[3];
+  }
+  const double* tgeo = A.test_geo + (size_t)il * kGeoStride;
+  const double nx[3] = {tgeo[9], tgeo[10], tgeo[11]};
+  const double ct[3] = {tgeo[kGeoCentroid], tgeo[kGeoCentroid + 1], tgeo[kGeoCentroid + 2]};
+  const int tv0 = A.test_vid[4 * il], tv1 = A.test_vid[4 * il + 1], tv2 = A.test_vid[4 * il + 2];
+  int lt[3] = {0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    if (a < A.nbt) lt[a] = A.test_ldof[(size_t)il * A.nbt + a];
+  const auto trial_pts = constant_view(A.trial_pts);
+  const auto trial_geo = constant_view(A.trial_geo);
+  const auto trial_vid = constant_view(A.trial_vid);
+  const auto trial_ldof = constant_view(A.trial_ldof);
+#pragma unroll 1
+  for (int jj = 0; jj < 16; ++jj) {
+    const int j = tile.y * kTile + wave * 16 + jj;
+    if (j >= A.n_trial) break;  // wave-uniform
+    const auto spp = trial_pts + (size_t)j * Q * 4;
+    double sp[Q * 4];
+#pragma unroll
+    for (int m = 0; m < Q * 4; ++m) sp[m] = spp[m];
+    const auto sg = trial_geo + (size_t)j * kGeoStride;
+    const int sv0 = trial_vid[4 * j], sv1 = trial_vid[4 * j + 1], sv2 = trial_vid[4 * j + 2];
+    int ls[3] = {0, 0, 0};
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+      if (b < A.nbs) ls[b] = trial_ldof[(size_t)j * A.nbs + b];
+    const bool adjacent = A.same_grid && (tv0 == sv0 || tv0 == sv1 || tv0 == sv2 || tv1 == sv0 || tv1 == sv1 ||
+                                          tv1 == sv2 || tv2 == sv0 || tv2 == sv1 || tv2 == sv2);
+    if (adjacent || !i_ok) continue;
+    cplx M[3][3], msum;
+    p1_local<Q, OP, HELM, CENTRED>(A, xp, wp, nx, ct, sp, sg, M, msum);
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        if (a >= A.nbt || b >= A.nbs) continue;  // uniform; constant indices keep M in registers
+        const cplx v = local_entry<OP, HELM>(M, a, b, msum, A.nbt, A.nbs, tgeo, sg, a, b, A.k);
+        double2* slot = acc + (ls[b] * ndt + lt[a]);
+        atomicAdd(&slot->x, v.re);
+        if constexpr (HELM) atomicAdd(&slot->y, v.im);
+      }
+  }
+  __syncthreads();
+  const bool mirror = symmetric && tile.x != tile.y;
+  for (int e = threadIdx.x; e < nacc; e += blockDim.x) {
+    const double2 v = acc[e];
+    if (v.x == 0.0 && v.y == 0.0) continue;
+    const int lsi = e / ndt, lti = e - lsi * ndt;
+    const long long row = A.test_blk_dofs[t0 + lti], col = A.trial_blk_dofs[s0 + lsi];
+    store_value(A.out, row + A.ld * col, {v.x, v.y}, A.out_type, kAtomic);
+    if (mirror) store_value(A.out, col + A.ld * row, {v.x, v.y}, A.out_type, kAtomic);
   }
 }
 
@@ -354,8 +463,6 @@ __global__ void __launch_bounds__(256) regular_kernel(const RegularArgs A) {
 // symmetric and both cells use the same rule. The transpose goes through LDS
 // so the mirrored stores are 128-byte contiguous segments.
 // ---------------------------------------------------------------------------
-constexpr int kTile = 64;
-constexpr int kChunk = 8;
 
 template <int Q, int OP, bool HELM, int CENTRED>
 __device__ __forceinline__ cplx p0_pair(const double (&xp)[Q][3], const double (&wp)[Q], const double* nx,
