@@ -1,2 +1,2 @@
 """Assembly (python/bempp/assembly of the reference)."""
-from . import batch, boundary  # noqa: F401
+from . import batch, boundary, potential  # noqa: F401

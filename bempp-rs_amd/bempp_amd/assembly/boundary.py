@@ -224,6 +224,14 @@ class BoundaryAssembler:
                                                                       test_space._rs_space))
         return _convert_to_scipy(m, self.dtype)
 
+    def assemble_singular_correction(self, trial_space: FunctionSpace, test_space: FunctionSpace) -> csr_matrix:
+        """Regular-rule contributions of the adjacent cell pairs as a CSR matrix
+        (python/bempp/assembly/boundary.py:157-169): dense = regular(all pairs) - this + singular."""
+        self._check_spaces(trial_space, test_space)
+        m = L.check_ptr(L.lib().boundary_assembler_assemble_singular_correction(
+            self._rs_assembler, trial_space._rs_space, test_space._rs_space))
+        return _convert_to_scipy(m, self.dtype)
+
     def apply(self, trial_space, test_space, u, transpose=False):
         """y = A u (A^T u with transpose) without assembling A (P0 spaces): the operator is
         recomputed on the GPU."""

@@ -309,6 +309,36 @@ MatrixT* boundary_assembler_assemble(const BoundaryAssemblerT* a, const SpaceT* 
       (MatrixT*)nullptr);
 }
 
+namespace {
+
+MatrixT* coo_to_matrix(const BoundaryAssemblerT* a, const SpaceT* trial, const SpaceT* test,
+                       const std::vector<int64_t>& rows, const std::vector<int64_t>& cols,
+                       const std::vector<double>& vals) {
+  auto* m = new MatrixT();
+  m->csr = true;
+  m->dtype = a->a->dtype;
+  m->shape0 = (size_t)test->s->size;
+  m->shape1 = (size_t)trial->s->size;
+  switch (m->dtype) {
+    case kF64: build_csr<double>(m, rows, cols, vals.data()); break;
+    case kC64: build_csr<std::complex<double>>(m, rows, cols, (const std::complex<double>*)vals.data()); break;
+    case kF32: {
+      std::vector<float> v(vals.begin(), vals.end());
+      build_csr<float>(m, rows, cols, v.data());
+      break;
+    }
+    default: {
+      std::vector<std::complex<float>> v(vals.size() / 2);
+      for (size_t i = 0; i < v.size(); ++i) v[i] = {(float)vals[2 * i], (float)vals[2 * i + 1]};
+      build_csr<std::complex<float>>(m, rows, cols, v.data());
+      break;
+    }
+  }
+  return m;
+}
+
+}  // namespace
+
 MatrixT* boundary_assembler_assemble_singular(const BoundaryAssemblerT* a, const SpaceT* trial, const SpaceT* test) {
   return guarded(
       [&]() -> MatrixT* {
@@ -316,27 +346,44 @@ MatrixT* boundary_assembler_assemble_singular(const BoundaryAssemblerT* a, const
         std::vector<int64_t> rows, cols;
         std::vector<double> vals;
         assemble_singular_coo(*a->a, *trial->s, *test->s, rows, cols, vals);
-        auto* m = new MatrixT();
-        m->csr = true;
-        m->dtype = a->a->dtype;
-        m->shape0 = (size_t)test->s->size;
-        m->shape1 = (size_t)trial->s->size;
-        switch (m->dtype) {
-          case kF64: build_csr<double>(m, rows, cols, vals.data()); break;
-          case kC64: build_csr<std::complex<double>>(m, rows, cols, (const std::complex<double>*)vals.data()); break;
-          case kF32: {
-            std::vector<float> v(vals.begin(), vals.end());
-            build_csr<float>(m, rows, cols, v.data());
-            break;
-          }
-          default: {
-            std::vector<std::complex<float>> v(vals.size() / 2);
-            for (size_t i = 0; i < v.size(); ++i) v[i] = {(float)vals[2 * i], (float)vals[2 * i + 1]};
-            build_csr<std::complex<float>>(m, rows, cols, v.data());
-            break;
-          }
-        }
-        return m;
+        return coo_to_matrix(a, trial, test, rows, cols, vals);
+      },
+      (MatrixT*)nullptr);
+}
+
+int potential_assemble_into_memory(const BoundaryAssemblerT* a, const SpaceT* space, const double* points,
+                                   size_t npoints, void* data) {
+  return guarded(
+      [&]() {
+        need(a, "assembler");
+        need(space, "space");
+        assemble_potential_host(*a->a, *space->s, points, (int64_t)npoints, data);
+        return 0;
+      },
+      1);
+}
+
+int potential_assemble_into_device_memory(const BoundaryAssemblerT* a, const SpaceT* space, const double* points,
+                                          size_t npoints, void* data, int accumulate, void* stream) {
+  return guarded(
+      [&]() {
+        need(a, "assembler");
+        need(space, "space");
+        assemble_potential_device(*a->a, *space->s, points, (int64_t)npoints, data, accumulate != 0, stream);
+        return 0;
+      },
+      1);
+}
+
+MatrixT* boundary_assembler_assemble_singular_correction(const BoundaryAssemblerT* a, const SpaceT* trial,
+                                                         const SpaceT* test) {
+  return guarded(
+      [&]() -> MatrixT* {
+        check_spaces(a, trial, test);
+        std::vector<int64_t> rows, cols;
+        std::vector<double> vals;
+        assemble_singular_correction_coo(*a->a, *trial->s, *test->s, rows, cols, vals);
+        return coo_to_matrix(a, trial, test, rows, cols, vals);
       },
       (MatrixT*)nullptr);
 }

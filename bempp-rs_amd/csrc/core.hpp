@@ -147,6 +147,17 @@ void assemble_host(const Assembler& a, const FunctionSpace& trial, const Functio
 // in the assembler dtype (interleaved for complex).
 void assemble_singular_coo(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test,
                            std::vector<int64_t>& rows, std::vector<int64_t>& cols, std::vector<double>& vals);
+// Regular-rule contributions of the adjacent pairs (the "singular correction") as COO.
+void assemble_singular_correction_coo(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test,
+                                      std::vector<int64_t>& rows, std::vector<int64_t>& cols,
+                                      std::vector<double>& vals);
+// Potential operator [npoints x ndofs] (column-major) of a single/double-layer assembler
+// on `space` at the given points; device variant (points and output in device memory,
+// on `stream`) and host variant (accumulates into a host buffer).
+void assemble_potential_device(const Assembler& a, const FunctionSpace& space, const double* dev_points,
+                               int64_t npoints, void* dev_out, bool accumulate, void* stream);
+void assemble_potential_host(const Assembler& a, const FunctionSpace& space, const double* points, int64_t npoints,
+                             void* host_out);
 // y = A u (or A^T u) without assembling A (P0 x P0 spaces); u, y in the assembler dtype.
 void apply(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test, const void* u, void* y,
            bool transpose = false);

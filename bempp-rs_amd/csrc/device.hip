@@ -868,6 +868,188 @@ void assemble_singular_coo(const Assembler& a, const FunctionSpace& trial, const
       }
 }
 
+template <int OP, bool HELM>
+void launch_correction_q(int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& R, const int4* pairs, int n,
+                         double* out) {
+  switch (Q) {
+    case 1: dev::correction_kernel<1, OP, HELM><<<grid, 256, 0, s>>>(R, pairs, n, out); break;
+    case 3: dev::correction_kernel<3, OP, HELM><<<grid, 256, 0, s>>>(R, pairs, n, out); break;
+    case 6: dev::correction_kernel<6, OP, HELM><<<grid, 256, 0, s>>>(R, pairs, n, out); break;
+    case 7: dev::correction_kernel<7, OP, HELM><<<grid, 256, 0, s>>>(R, pairs, n, out); break;
+    default: throw Error("unsupported regular quadrature point count");
+  }
+}
+
+template <bool HELM>
+void launch_correction(int op, int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& R, const int4* pairs, int n,
+                       double* out) {
+  switch (op) {
+    case kSL: launch_correction_q<kSL, HELM>(Q, grid, s, R, pairs, n, out); break;
+    case kDL: launch_correction_q<kDL, HELM>(Q, grid, s, R, pairs, n, out); break;
+    case kADL: launch_correction_q<kADL, HELM>(Q, grid, s, R, pairs, n, out); break;
+    case kHS: launch_correction_q<kHS, HELM>(Q, grid, s, R, pairs, n, out); break;
+    default: throw Error("operator not supported");
+  }
+}
+
+// Regular-rule contributions of the adjacent pairs as COO triplets (same layout as
+// assemble_singular_coo, cells' own local basis order).
+void assemble_singular_correction_coo(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test,
+                                      std::vector<int64_t>& rows, std::vector<int64_t>& cols,
+                                      std::vector<double>& vals) {
+  rows.clear();
+  cols.clear();
+  vals.clear();
+  if (test.grid != trial.grid) return;  // no adjacent pairs across grids
+  Prepared P = prepare(a, trial, test, nullptr, nullptr);
+  if (P.npairs == 0) return;
+  const int nbl = test.nb * trial.nb;
+  const size_t nvals = (size_t)P.npairs * nbl;
+  double* d = nullptr;
+  HIP_CHECK(hipMalloc((void**)&d, nvals * 2 * sizeof(double)));
+  std::vector<double> h(nvals * 2);
+  try {
+    const dim3 grid((unsigned)((P.npairs + 255) / 256));
+    if (a.kind == kHelmholtz) launch_correction<true>(a.op, P.Q, grid, P.stream, P.R, P.S.pairs, P.npairs, d);
+    else launch_correction<false>(a.op, P.Q, grid, P.stream, P.R, P.S.pairs, P.npairs, d);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(h.data(), d, h.size() * sizeof(double), hipMemcpyDeviceToHost, P.stream));
+    HIP_CHECK(hipStreamSynchronize(P.stream));
+  } catch (...) {
+    (void)hipFree(d);
+    throw;
+  }
+  HIP_CHECK(hipFree(d));
+  const auto& pairs = const_cast<Grid&>(*test.grid).device_cache.at(P.device)->host_pairs;
+  rows.resize(nvals);
+  cols.resize(nvals);
+  const bool cplx = is_complex(a.dtype);
+  vals.resize(nvals * (cplx ? 2 : 1));
+  for (size_t p = 0; p < pairs.size(); ++p)
+    for (int b = 0; b < trial.nb; ++b)
+      for (int t = 0; t < test.nb; ++t) {
+        const size_t e = p * nbl + t + test.nb * b;
+        rows[e] = test.dofs[(size_t)pairs[p].test * test.nb + t];
+        cols[e] = trial.dofs[(size_t)pairs[p].trial * trial.nb + b];
+        if (cplx) {
+          vals[2 * e] = h[2 * e];
+          vals[2 * e + 1] = h[2 * e + 1];
+        } else {
+          vals[e] = h[2 * e];
+        }
+      }
+}
+
+// ---- potential (off-surface) operators ----
+template <int OP, bool HELM>
+void launch_potential_q(int Q, dim3 grid, hipStream_t s, const dev::PotentialArgs& P) {
+  switch (Q) {
+    case 1: dev::potential_kernel<1, OP, HELM><<<grid, 256, 0, s>>>(P); break;
+    case 3: dev::potential_kernel<3, OP, HELM><<<grid, 256, 0, s>>>(P); break;
+    case 6: dev::potential_kernel<6, OP, HELM><<<grid, 256, 0, s>>>(P); break;
+    case 7: dev::potential_kernel<7, OP, HELM><<<grid, 256, 0, s>>>(P); break;
+    default: throw Error("unsupported regular quadrature point count");
+  }
+}
+
+void assemble_potential_device(const Assembler& a, const FunctionSpace& space_c, const double* dev_points,
+                               int64_t npoints, void* dev_out, bool accumulate, void* stream) {
+  if (a.op != kSL && a.op != kDL) throw Error("potential operators exist for the single and double layer only");
+  if ((a.kind == kLaplace) == is_complex(a.dtype)) throw Error("dtype does not match the kernel");
+  if (npoints < 0) throw Error("negative point count");
+  if (npoints == 0) return;
+  if (!dev_points || !dev_out) throw Error("points or output pointer is null");
+  auto& space = const_cast<FunctionSpace&>(space_c);
+  auto& grid = const_cast<Grid&>(*space.grid);
+  const int device = current_device();
+  hipStream_t s = (hipStream_t)stream;
+  const int Q = regular_points(a.options);
+  std::vector<double> xi, w;
+  triangle_rule(Q, xi, w);
+  DeviceGridCache* gc = grid_cache(grid, device);
+  dev::PotentialArgs P{};
+  P.x = dev_points;
+  P.np = npoints;
+  P.trial_pts = grid_points_for_rule(grid, gc, Q, xi, w);
+  P.trial_geo = gc->geo;
+  P.trial_dofs = space_dofs(space, device);
+  P.n_trial = (int)grid.nc();
+  P.nbs = space.nb;
+  P.k = a.k;
+  for (int p = 0; p < dev::kMaxQ; ++p)
+    for (int b = 0; b < 3; ++b) P.phi[p][b] = 0.0;
+  for (int p = 0; p < Q; ++p) {
+    P.phi[p][0] = 1.0 - xi[2 * p] - xi[2 * p + 1];
+    P.phi[p][1] = xi[2 * p];
+    P.phi[p][2] = xi[2 * p + 1];
+  }
+  P.out = dev_out;
+  P.out_type = out_type(a.dtype);
+  P.mode = space.nb == 1 ? dev::kAccumulate : dev::kAtomic;
+  if (!accumulate) {
+    const long long n_out = (long long)npoints * (long long)space.size * (is_complex(a.dtype) ? 2 : 1);
+    if (a.dtype == kF32 || a.dtype == kC32) HIP_CHECK(hipMemsetAsync(dev_out, 0, (size_t)n_out * 4, s));
+    else dev::fill_zero_kernel<<<2048, 256, 0, s>>>((double*)dev_out, n_out);
+    HIP_CHECK(hipGetLastError());
+  }
+  const long long bx = (npoints + 255) / 256;
+  if (bx > INT32_MAX) throw Error("too many evaluation points");
+  long long chunk = (bx * P.n_trial + 16383) / 16384;
+  chunk = std::max(4LL, std::min(128LL, chunk));
+  chunk = std::max(chunk, ((long long)P.n_trial + 65534) / 65535);
+  P.trial_chunk = (int)chunk;
+  const dim3 g((unsigned)bx, (unsigned)((P.n_trial + chunk - 1) / chunk));
+  const bool helm = a.kind == kHelmholtz;
+  if (a.op == kSL) {
+    if (helm) launch_potential_q<kSL, true>(Q, g, s, P);
+    else launch_potential_q<kSL, false>(Q, g, s, P);
+  } else {
+    if (helm) launch_potential_q<kDL, true>(Q, g, s, P);
+    else launch_potential_q<kDL, false>(Q, g, s, P);
+  }
+  HIP_CHECK(hipGetLastError());
+}
+
+void assemble_potential_host(const Assembler& a, const FunctionSpace& space, const double* points, int64_t npoints,
+                             void* host_out) {
+  if (npoints == 0) return;
+  if (!points || !host_out) throw Error("points or output pointer is null");
+  if (a.kind == kHelmholtz) {  // the device sine/cosine reduction is exact while k r <= 1e8
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    auto grow = [&](const double* x) {
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = std::min(lo[d], x[d]);
+        hi[d] = std::max(hi[d], x[d]);
+      }
+    };
+    for (int64_t v = 0; v < space.grid->nv(); ++v) grow(&space.grid->points[3 * v]);
+    for (int64_t j = 0; j < npoints; ++j) grow(points + 3 * j);
+    double diag = 0.0;
+    for (int d = 0; d < 3; ++d) diag += (hi[d] - lo[d]) * (hi[d] - lo[d]);
+    if (!(std::fabs(a.k) * std::sqrt(diag) <= 1e8)) throw Error("|k| times the extent of mesh and points exceeds 1e8");
+  }
+  const int device = current_device();
+  const size_t bytes = (size_t)npoints * (size_t)space.size * dtype_size(a.dtype);
+  void* d = nullptr;
+  double* dp = nullptr;
+  HIP_CHECK(hipMalloc(&d, std::max<size_t>(bytes, 1)));
+  try {
+    HIP_CHECK(hipMalloc((void**)&dp, (size_t)npoints * 3 * sizeof(double)));
+    hipStream_t s = assembler_stream(a, device);
+    HIP_CHECK(hipMemcpyAsync(dp, points, (size_t)npoints * 3 * sizeof(double), hipMemcpyHostToDevice, s));
+    if (bytes) HIP_CHECK(hipMemcpyAsync(d, host_out, bytes, hipMemcpyHostToDevice, s));
+    assemble_potential_device(a, space, dp, npoints, d, true, s);
+    if (bytes) HIP_CHECK(hipMemcpyAsync(host_out, d, bytes, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+  } catch (...) {
+    (void)hipFree(d);
+    if (dp) (void)hipFree(dp);
+    throw;
+  }
+  HIP_CHECK(hipFree(d));
+  HIP_CHECK(hipFree(dp));
+}
+
 Assembler::~Assembler() {
   for (void* e : events)
     if (e) (void)hipEventDestroy((hipEvent_t)e);

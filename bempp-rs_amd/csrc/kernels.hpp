@@ -760,6 +760,124 @@ __global__ void __launch_bounds__(256) batched_singular_kernel(const SingularArg
     singular_pair<OP, HELM, true, false>(A, g - fv[lo], lane);
 }
 
+// Singular correction: the regular-rule sum over each adjacent cell pair (the part of
+// a regular-rule evaluation over all point pairs -- e.g. by a fast multipole method --
+// that the Duffy rules replace), point pairs at zero distance (the same point of a
+// coincident pair) contributing nothing. One thread per adjacent pair; COO output of
+// nbt x nbs entries per pair (cells' own local basis order), interleaved complex.
+template <int Q, int OP, bool HELM>
+__global__ void __launch_bounds__(256) correction_kernel(const RegularArgs A, const int4* __restrict__ pairs,
+                                                         int npairs, double* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npairs) return;
+  const int4 pr = pairs[t];
+  const int c = pr.x, d = pr.y;
+  const double* tgeo = A.test_geo + (size_t)c * kGeoStride;
+  const double* sgeo = A.trial_geo + (size_t)d * kGeoStride;
+  const double nx[3] = {tgeo[9], tgeo[10], tgeo[11]};
+  const double ny[3] = {sgeo[9], sgeo[10], sgeo[11]};
+  cplx M[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) M[a][b] = {0.0, 0.0};
+  for (int p = 0; p < Q; ++p) {
+    const double* xs = A.test_pts + ((size_t)c * Q + p) * 4;
+    for (int q = 0; q < Q; ++q) {
+      const double* ys = A.trial_pts + ((size_t)d * Q + q) * 4;
+      const double dx = xs[0] - ys[0], dy = xs[1] - ys[1], dz = xs[2] - ys[2];
+      if (dx == 0.0 && dy == 0.0 && dz == 0.0) continue;
+      const cplx kv = kernel_scalar<OP, HELM, 0>(dx, dy, dz, A.k, nx, ny);
+      const double w = xs[3] * ys[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const double fa = (A.nbt == 3 ? A.phi[p][a] : (a == 0 ? 1.0 : 0.0)) * w;
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          const double f = fa * (A.nbs == 3 ? A.phi[q][b] : (b == 0 ? 1.0 : 0.0));
+          M[a][b].re = fma(kv.re, f, M[a][b].re);
+          M[a][b].im = fma(kv.im, f, M[a][b].im);
+        }
+      }
+    }
+  }
+  cplx msum = {0.0, 0.0};
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      msum.re += M[a][b].re;
+      msum.im += M[a][b].im;
+    }
+  const int nbl = A.nbt * A.nbs;
+#pragma unroll
+  for (int b = 0; b < 3; ++b)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (a >= A.nbt || b >= A.nbs) continue;
+      const cplx v = local_entry<OP, HELM>(M, a, b, msum, A.nbt, A.nbs, tgeo, sgeo, a, b, A.k);
+      const long long e = (long long)t * nbl + a + A.nbt * b;
+      out[2 * e] = v.re;
+      out[2 * e + 1] = v.im;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Potential (off-surface) operators: P[j][i] = sum_cells sum_q K(x_j, y_q) psi_i(y_q) w_q,
+// K = G (single layer) or grad_y G . n_y (double layer). One thread per evaluation point
+// (lane), the block walks a chunk of trial cells with wave-uniform scalar loads.
+// ---------------------------------------------------------------------------
+struct PotentialArgs {
+  const double* x;          // [np][3] evaluation points
+  long long np;
+  const double* trial_pts;  // [n_trial][Q][4]
+  const double* trial_geo;  // [n_trial][kGeoStride]
+  const int* trial_dofs;    // [n_trial][nbs]
+  int n_trial, nbs, trial_chunk;
+  double k;
+  double phi[kMaxQ][3];
+  void* out;                // [np x ndofs] column-major
+  int out_type;
+  int mode;                 // kAccumulate (P0: one thread per entry) or kAtomic (P1)
+};
+
+template <int Q, int OP, bool HELM>
+__global__ void __launch_bounds__(256) potential_kernel(const PotentialArgs A) {
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const bool ok = t < A.np;
+  const long long tl = ok ? t : A.np - 1;
+  const double x[3] = {A.x[3 * tl], A.x[3 * tl + 1], A.x[3 * tl + 2]};
+  const auto trial_pts = constant_view(A.trial_pts);
+  const auto trial_geo = constant_view(A.trial_geo);
+  const auto trial_dofs = constant_view(A.trial_dofs);
+  const int j0 = blockIdx.y * A.trial_chunk;
+  const int j1 = min(j0 + A.trial_chunk, A.n_trial);
+  for (int j = j0; j < j1; ++j) {
+    const auto sp = trial_pts + (size_t)j * Q * 4;
+    const auto sg = trial_geo + (size_t)j * kGeoStride;
+    const double ny[3] = {sg[9], sg[10], sg[11]};
+    cplx T[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const cplx kv = kernel_scalar<OP, HELM, 0>(x[0] - sp[4 * q], x[1] - sp[4 * q + 1], x[2] - sp[4 * q + 2], A.k,
+                                                 ny, ny);
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        const double f = (A.nbs == 3 ? A.phi[q][b] : (b == 0 ? 1.0 : 0.0)) * sp[4 * q + 3];
+        T[b].re = fma(kv.re, f, T[b].re);
+        T[b].im = fma(kv.im, f, T[b].im);
+      }
+    }
+    if (!ok) continue;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      if (b >= A.nbs) continue;
+      const long long col = trial_dofs[(size_t)j * A.nbs + b];
+      store_value(A.out, t + A.np * col, {T[b].re * kInv4Pi, T[b].im * kInv4Pi}, A.out_type, A.mode);
+    }
+  }
+}
+
 // (a template so that every translation unit including this header may instantiate it)
 template <typename T = double>
 __global__ void fill_zero_kernel(T* p, long long n) {

@@ -177,6 +177,32 @@ MatrixT* boundary_assembler_assemble(const BoundaryAssemblerT* assembler, const 
 MatrixT* boundary_assembler_assemble_singular(const BoundaryAssemblerT* assembler, const SpaceT* trial_space,
                                               const SpaceT* test_space);
 
+/* Singular correction (python/bempp/assembly/boundary.py:157-169 binds
+ * boundary_assembler_assemble_singular_correction; the Rust side has no implementation):
+ * CSR matrix of the regular-rule contributions of the adjacent pairs, i.e. what a
+ * regular-rule evaluation over all point pairs (an FMM, tests/fmm.rs.bak:22-95) adds at
+ * the pairs the Duffy rules integrate, so that dense = regular(all) - correction +
+ * singular. Point pairs at zero distance contribute nothing. Same CSR conventions as
+ * assemble_singular. */
+MatrixT* boundary_assembler_assemble_singular_correction(const BoundaryAssemblerT* assembler,
+                                                         const SpaceT* trial_space, const SpaceT* test_space);
+
+/* Potential (off-surface) operators -- the PotentialAssembler of the reference's commented
+ * tests (tests/compare_to_bempp_cl.rs:208-320; src/boundary_evaluators.rs is empty there):
+ * P[j][i] = integral over the grid of K(x_j, y) psi_i(y), K = G (single layer) or
+ * grad_y G . n_y (double layer), with the assembler's kernel, wavenumber, dtype and regular
+ * rule; psi_i the basis of `space`; points [npoints][3] f64. The matrix is npoints x ndofs,
+ * Fortran order (entry (j, i) at data[j + npoints * i]).
+ * into_memory accumulates into a host buffer; requires |k| times the extent of mesh and
+ * points <= 1e8. */
+int potential_assemble_into_memory(const BoundaryAssemblerT* assembler, const SpaceT* space, const double* points,
+                                   size_t npoints, void* data);
+/* Device variant (no counterpart): points and data in device memory on the current device,
+ * accumulate != 0 adds, otherwise overwrites; enqueued on `stream` (NULL: null stream). */
+int potential_assemble_into_device_memory(const BoundaryAssemblerT* assembler, const SpaceT* space,
+                                          const double* points, size_t npoints, void* data, int accumulate,
+                                          void* stream);
+
 /* Matrix-free operator application y = A u (no counterpart in the reference; the
  * FMM prototype of tests/fmm.rs.bak applies operators this way): A is recomputed on
  * the device and never stored. P0 spaces; u has n_trial and y n_test entries of the

@@ -816,6 +816,58 @@ int oracle_space(const double* pts, int64_t nv, const int64_t* cells, int64_t nc
   }
 }
 
+// Potential (off-surface) operator, accumulated into out [npoints x ndofs] column-major
+// (entry (point j, dof i) at out[j + npoints i]; complex interleaved for Helmholtz):
+//   P_ji = sum over cells c, regular points q: K(x_j, y_q) phi_i(y_q) w_q jdet_c,
+//   single layer K = G(x, y), double layer K = grad_y G(x, y) . n_y.
+// The reference declares the PotentialAssembler (commented tests at
+// tests/compare_to_bempp_cl.rs:208-320, bempp-cl goldens) with src/boundary_evaluators.rs
+// left empty; this follows the same regular-rule evaluation as the boundary operators.
+int oracle_potential(int kind, int op, double k, const double* pts, int64_t nv, const int64_t* cells, int64_t nc,
+                     int degree, int cont, int nq, const double* xs, int64_t np, void* out) {
+  try {
+    if (op != 0 && op != 1) throw std::runtime_error("potentials: op must be 0 (single) or 1 (double layer)");
+    Grid g(pts, nv, cells, nc);
+    Space sp(&g, degree, cont != 0);
+    std::vector<double> rp, rw;
+    regular_rule(nq, rp, rw);
+    const int Q = (int)rw.size();
+    const bool helm = kind == 1;
+    double* outd = (double*)out;
+    const int stride = helm ? 2 : 1;
+#pragma omp parallel for schedule(static)
+    for (int64_t j = 0; j < np; ++j) {
+      const double* x = xs + 3 * j;
+      for (int64_t c = 0; c < nc; ++c) {
+        const CellGeom G = cell_geom(g, c);
+        for (int q = 0; q < Q; ++q) {
+          double y[3];
+          map_point(G, rp[2 * q], rp[2 * q + 1], y);
+          const double d[3] = {x[0] - y[0], x[1] - y[1], x[2] - y[2]};
+          const double r = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+          const double g0 = 1.0 / (4.0 * kPi * r);
+          cplx val = helm ? cplx(std::cos(k * r), std::sin(k * r)) * g0 : cplx(g0, 0.0);
+          if (op == 1) {
+            const cplx f = helm ? val * cplx(1.0, -k * r) / (r * r) : val / (r * r);
+            val = f * (d[0] * G.n[0] + d[1] * G.n[1] + d[2] * G.n[2]);
+          }
+          const double w = rw[q] * G.jdet;
+          for (int b = 0; b < sp.nb; ++b) {
+            const double phi = sp.nb == 1 ? 1.0 : (b == 0 ? 1.0 - rp[2 * q] - rp[2 * q + 1] : rp[2 * q + b - 1]);
+            const int64_t e = j + np * sp.dofs[sp.nb * c + b];
+            outd[stride * e] += val.real() * phi * w;
+            if (helm) outd[stride * e + 1] += val.imag() * phi * w;
+          }
+        }
+      }
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
 // Regular rule (pts [2Q], wts [Q]); returns Q or -1.
 int oracle_regular_rule(int npts, double* pts, double* wts) {
   try {

@@ -32,6 +32,7 @@ def lib():
         L.oracle_regular_rule.argtypes = [i32, vp, vp]
         L.oracle_gauss_legendre01.argtypes = [i32, vp, vp]
         L.oracle_duffy_rule.argtypes = [i32, vp, vp, i32, vp, vp, vp]
+        L.oracle_potential.argtypes = [i32, i32, dbl, vp, i64, vp, i64, i32, i32, i32, vp, i64, vp]
         _lib = L
     return _lib
 
@@ -106,6 +107,18 @@ def assemble_singular_coo(kind, op, pts, cells, test=(0, False), trial=(0, False
     vals = np.zeros(n, dtype=np.float64 if kind == "laplace" else np.complex128)
     _check(lib().oracle_assemble_singular_coo(*args, ctypes.byref(cnt), _p(rows), _p(cols), _p(vals)))
     return rows, cols, vals
+
+
+def potential(kind, op, pts, cells, xs, space=(0, False), k=0.0, nq=6):
+    """Potential operator matrix [npoints, ndofs] (see oracle_potential)."""
+    pts = np.ascontiguousarray(pts, dtype=np.float64)
+    cells = np.ascontiguousarray(cells, dtype=np.int64)
+    xs = np.ascontiguousarray(xs, dtype=np.float64).reshape(-1, 3)
+    ndofs = _space_size(pts, cells, *space)
+    out = np.zeros((len(xs), ndofs), dtype=np.float64 if kind == "laplace" else np.complex128, order="F")
+    _check(lib().oracle_potential(KINDS[kind], OPS[op], float(k), _p(pts), len(pts), _p(cells), len(cells),
+                                  space[0], int(space[1]), nq, _p(xs), len(xs), _p(out)))
+    return out
 
 
 def regular_rule(npts):

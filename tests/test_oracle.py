@@ -244,3 +244,54 @@ def test_row_restriction():
     assert np.abs(part - full[17:45]).max() < 1e-16
     fast = O.assemble("helmholtz", "single_layer", p, c, k=5.0, fast=True)
     assert np.abs(fast - full).max() / np.abs(full).max() < 1e-14
+
+
+@pytest.mark.parametrize("kind,op,k", [("laplace", "single_layer", 0.0), ("helmholtz", "double_layer", 3.0)])
+@pytest.mark.parametrize("deg", [(0, False), (1, True)])
+def test_fmm_identity_on_oracle(kind, op, k, deg):
+    """tests/fmm.rs.bak:22-95: dense = P^T K P - C + S, with K the kernel on all regular
+    points (zero at zero distance), C the regular-rule sums of the adjacent pairs and S the
+    Duffy part. Pins the oracle's split of the dense matrix into the two loops."""
+    import fmm_helpers as F
+
+    pts, cells = O.sphere(1, "icosahedron")
+    ndofs, dofs, _, _ = O.space(pts, cells, *deg)
+    PKP, C = F.regular_parts(kind, op, k, pts, cells, dofs, ndofs)
+    rows, cols, vals = O.assemble_singular_coo(kind, op, pts, cells, test=deg, trial=deg, k=k)
+    S = np.zeros((ndofs, ndofs), dtype=vals.dtype)
+    np.add.at(S, (rows, cols), vals)
+    dense = O.assemble(kind, op, pts, cells, test=deg, trial=deg, k=k)
+    assert np.linalg.norm(PKP - C + S - dense) / np.linalg.norm(dense) < 1e-12
+
+
+def test_potential_goldens_oracle():
+    """The oracle's potential operators reproduce the four bempp-cl golden potentials of
+    tests/compare_to_bempp_cl.rs:208-320 (DP0, octahedron level 0, points 2 e_i) with the
+    6-point rule."""
+    import json
+    from pathlib import Path
+
+    cases = json.loads((Path(__file__).parent / "golden" / "bempp_cl_potentials_octahedron_l0.json").read_text())
+    pts, cells = O.sphere(0, "octahedron")
+    for c in cases:
+        ref = np.array(c["from_cl"])
+        if ref.ndim == 3:
+            ref = ref[..., 0] + 1j * ref[..., 1]
+        P = O.potential(c["kind"], c["operator"], pts, cells, np.array(c["points"]), k=c["k"], nq=6)
+        assert np.abs(P.T - ref).max() < 1e-14, c["name"]
+
+
+def test_potential_identities_oracle():
+    """Gauss' law for the double-layer potential of a unit density (-1 inside, 0 outside,
+    exact for any closed flat-triangle surface up to quadrature error) and the exterior
+    single-layer potential R^2/|x| of a unit density on the sphere (converging with h^2)."""
+    inside = np.array([[0.0, 0.0, 0.0], [0.0, 0.3, 0.2]])
+    outside = np.array([[2.0, 0.0, 0.0], [0.0, 3.0, 0.0]])
+    errs = []
+    for level in (2, 3):
+        pts, cells = O.sphere(level, "icosahedron")
+        D = O.potential("laplace", "double_layer", pts, cells, np.vstack([inside, outside]))
+        assert np.allclose(D.sum(axis=1), [-1, -1, 0, 0], atol=1e-6)
+        S = O.potential("laplace", "single_layer", pts, cells, outside).sum(axis=1)
+        errs.append(np.abs(S - 1.0 / np.linalg.norm(outside, axis=1)).max())
+    assert errs[1] < errs[0] / 3.5  # O(h^2)
