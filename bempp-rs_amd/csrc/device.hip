@@ -621,8 +621,8 @@ Prepared prepare(const Assembler& a, const FunctionSpace& trial_c, const Functio
   dev::RegularArgs& R = P.R;
   R.test_pts = grid_points_for_rule(tgrid, tc, P.Q, xi, w);
   R.trial_pts = grid_points_for_rule(sgrid, sc, P.Q, xi, w);
-  // mixed precision: P0 x P0 on one grid
-  P.mixed = a.options.precision == kMixed && test.nb == 1 && trial.nb == 1 && P.same_grid;
+  // mixed precision: one grid (P0 x P0: p0 mixed tiles; a P1 side: LDS mixed tiles)
+  P.mixed = a.options.precision == kMixed && P.same_grid;
   P.near = nullptr;
   P.nnear = 0;
   R.test_pts32 = R.trial_pts32 = nullptr;
@@ -768,8 +768,12 @@ void assemble_device(const Assembler& a, const FunctionSpace& trial, const Funct
         const int nta = (R.n_test + dev::kTile - 1) / dev::kTile, ntb = (R.n_trial + dev::kTile - 1) / dev::kTile;
         int ntiles = 0;
         const int2* tiles = tile_list(P.device, nta, ntb, sym, &ntiles);
-        launch_regular(a.op, a.kind == kHelmholtz, P.tier, 2, P.Q, dim3(ntiles), P.stream, R, tiles, sym ? 1 : 0,
-                       lds);
+        if (P.mixed)
+          launch_mixed_lds(a.op, a.kind == kHelmholtz, P.tier, P.Q, dim3(ntiles), P.stream, R, tiles, sym ? 1 : 0,
+                           lds, P.near, P.nnear);
+        else
+          launch_regular(a.op, a.kind == kHelmholtz, P.tier, 2, P.Q, dim3(ntiles), P.stream, R, tiles,
+                         sym ? 1 : 0, lds);
       } else {
         launch_regular(a.op, a.kind == kHelmholtz, P.tier, 0, P.Q, grid, P.stream, R);
       }

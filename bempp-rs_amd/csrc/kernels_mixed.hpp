@@ -266,6 +266,208 @@ __global__ void __launch_bounds__(256) regular_p0_tiled_mixed(const RegularArgs 
   p0_tile_mixed<Q, OP, HELM, PH>(A, tiles[blockIdx.x], symmetric, lds);
 }
 
+// ---------------------------------------------------------------------------
+// Mixed precision with a P1 side: the LDS-accumulating tiles of regular_tiled_lds with
+// far cell pairs evaluated in FP32 (same per-point-pair arithmetic as p0_tile_mixed;
+// the trial basis sums T_b(p) = sum_q w_q K_pq psi_b(q) in FP32, the test basis sums and
+// e^{ikR} in FP64). Near and adjacent pairs contribute nothing here: near pairs come from
+// near_pair_p1_kernel in FP64, adjacent pairs from the Duffy kernel.
+// ---------------------------------------------------------------------------
+template <int Q, int OP, bool HELM, int PH>
+__global__ void __launch_bounds__(256) regular_tiled_lds_mixed(const RegularArgs A, const int2* __restrict__ tiles,
+                                                               int symmetric) {
+  constexpr bool kDN = OP == 1 || OP == 2;
+  extern __shared__ double2 acc[];
+  const int2 tile = tiles[blockIdx.x];
+  const int t0 = A.test_blk_off[tile.x], ndt = A.test_blk_off[tile.x + 1] - t0;
+  const int s0 = A.trial_blk_off[tile.y], nds = A.trial_blk_off[tile.y + 1] - s0;
+  const int nacc = ndt * nds;
+  for (int e = threadIdx.x; e < nacc; e += blockDim.x) acc[e] = make_double2(0.0, 0.0);
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i = tile.x * kTile + lane;
+  const bool i_ok = i < A.n_test;
+  const int il = i_ok ? i : A.n_test - 1;
+  float mp[Q][3], mq[Q];  // m = -2 o_p and |o_p|^2
+  double wp[Q];
+#pragma unroll
+  for (int p = 0; p < Q; ++p) {
+    const float4 o = ((const float4*)A.test_pts32)[(size_t)il * Q + p];
+    mp[p][0] = -2.0f * o.x;
+    mp[p][1] = -2.0f * o.y;
+    mp[p][2] = -2.0f * o.z;
+    mq[p] = fmaf(o.x, o.x, fmaf(o.y, o.y, o.z * o.z));
+    wp[p] = A.test_pts[((size_t)il * Q + p) * 4 + 3];
+  }
+  const double* tgeo = A.test_geo + (size_t)il * kGeoStride;
+  const float nx[3] = {(float)tgeo[9], (float)tgeo[10], (float)tgeo[11]};
+  const double ct[3] = {tgeo[kGeoCentroid], tgeo[kGeoCentroid + 1], tgeo[kGeoCentroid + 2]};
+  const double trad = tgeo[kGeoRadius];
+  const int tv0 = A.test_vid[4 * il], tv1 = A.test_vid[4 * il + 1], tv2 = A.test_vid[4 * il + 2];
+  int lt[3] = {0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    if (a < A.nbt) lt[a] = A.test_ldof[(size_t)il * A.nbt + a];
+  const float4* trial_pts32 = (const float4*)A.trial_pts32;
+  const auto trial_geo = constant_view(A.trial_geo);
+  const auto trial_vid = constant_view(A.trial_vid);
+  const auto trial_ldof = constant_view(A.trial_ldof);
+  float kf;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(kf) : "s"((float)A.k));
+#pragma unroll 1
+  for (int jj = 0; jj < 16; ++jj) {
+    const int j = tile.y * kTile + wave * 16 + jj;
+    if (j >= A.n_trial) break;  // wave-uniform
+    float sp[Q * 4];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const float4 t = trial_pts32[(size_t)j * Q + q];
+      sp[4 * q] = t.x;
+      sp[4 * q + 1] = t.y;
+      sp[4 * q + 2] = t.z;
+      sp[4 * q + 3] = t.w;
+    }
+    const auto sg = trial_geo + (size_t)j * kGeoStride;
+    const double cs[3] = {sg[kGeoCentroid], sg[kGeoCentroid + 1], sg[kGeoCentroid + 2]};
+    const double srad = sg[kGeoRadius];
+    const float ny[3] = {(float)sg[9], (float)sg[10], (float)sg[11]};
+    const int sv0 = trial_vid[4 * j], sv1 = trial_vid[4 * j + 1], sv2 = trial_vid[4 * j + 2];
+    int ls[3] = {0, 0, 0};
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+      if (b < A.nbs) ls[b] = trial_ldof[(size_t)j * A.nbs + b];
+    const bool adjacent = A.same_grid && (tv0 == sv0 || tv0 == sv1 || tv0 == sv2 || tv1 == sv0 || tv1 == sv1 ||
+                                          tv1 == sv2 || tv2 == sv0 || tv2 == sv1 || tv2 == sv2);
+    double R2;
+    const bool far = far_pair(ct, cs, trad, srad, &R2) && !adjacent && i_ok;
+    if (!far) continue;
+    const double Dx = ct[0] - cs[0], Dy = ct[1] - cs[1], Dz = ct[2] - cs[2];
+    const double R = R2 * rsqrt_nr(R2);
+    const float D[3] = {(float)Dx, (float)Dy, (float)Dz};
+    const float D2[3] = {2.0f * D[0], 2.0f * D[1], 2.0f * D[2]};
+    const float R2f = (float)R2, Rf = (float)R;
+    const float* nrm = OP == 1 ? ny : nx;
+    const float sgn = OP == 2 ? -1.0f : 1.0f;
+    double Dn64 = 0.0;
+    if constexpr (OP == 1) Dn64 = Dx * sg[9] + Dy * sg[10] + Dz * sg[11];
+    if constexpr (OP == 2) Dn64 = -(Dx * tgeo[9] + Dy * tgeo[10] + Dz * tgeo[11]);
+    const float Dn = (float)Dn64;
+    const float nh[3] = {-0.5f * sgn * nrm[0], -0.5f * sgn * nrm[1], -0.5f * sgn * nrm[2]};
+    float bq[Q], nq[Q], ph[Q][3];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const float ox = sp[4 * q], oy = sp[4 * q + 1], oz = sp[4 * q + 2];
+      bq[q] = fmaf(D2[0] - ox, ox, fmaf(D2[1] - oy, oy, (D2[2] - oz) * oz));
+      nq[q] = kDN ? sgn * fmaf(ox, nrm[0], fmaf(oy, nrm[1], oz * nrm[2])) : 0.0f;
+#pragma unroll
+      for (int b = 0; b < 3; ++b) ph[q][b] = A.nbs == 3 ? (float)A.phi[q][b] : (b == 0 ? 1.0f : 0.0f);
+    }
+    cplx M[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) M[a][b] = {0.0, 0.0};
+#pragma unroll
+    for (int p = 0; p < Q; ++p) {
+      const float ap = fmaf(-mp[p][0], D[0], fmaf(-mp[p][1], D[1], fmaf(-mp[p][2], D[2], mq[p])));
+      const float dnp = kDN ? fmaf(mp[p][0], nh[0], fmaf(mp[p][1], nh[1], fmaf(mp[p][2], nh[2], Dn))) : 0.0f;
+      cplxf T[3] = {{0.0f, 0.0f}, {0.0f, 0.0f}, {0.0f, 0.0f}};
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const float s = fmaf(mp[p][0], sp[4 * q], fmaf(mp[p][1], sp[4 * q + 1], fmaf(mp[p][2], sp[4 * q + 2], ap - bq[q])));
+        cplxf kw = {0.0f, 0.0f};
+        far_accumulate_f32<OP, HELM, PH>(s, dnp - nq[q], R2f, Rf, kf, sp[4 * q + 3], kw);
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          T[b].re = fmaf(kw.re, ph[q][b], T[b].re);
+          T[b].im = fmaf(kw.im, ph[q][b], T[b].im);
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const double f = (A.nbt == 3 ? A.phi[p][a] : (a == 0 ? 1.0 : 0.0)) * wp[p];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          M[a][b].re = fma(f, (double)T[b].re, M[a][b].re);
+          M[a][b].im = fma(f, (double)T[b].im, M[a][b].im);
+        }
+      }
+    }
+    cplx e = {1.0, 0.0};
+    if constexpr (HELM) sincos_cw(A.k * R, &e.im, &e.re);
+    cplx msum = {0.0, 0.0};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        if constexpr (HELM) M[a][b] = cmul(M[a][b], e);
+        msum.re += M[a][b].re;
+        msum.im += M[a][b].im;
+      }
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        if (a >= A.nbt || b >= A.nbs) continue;
+        const cplx v = local_entry<OP, HELM>(M, a, b, msum, A.nbt, A.nbs, tgeo, sg, a, b, A.k);
+        double2* slot = acc + (ls[b] * ndt + lt[a]);
+        atomicAdd(&slot->x, v.re);
+        if constexpr (HELM) atomicAdd(&slot->y, v.im);
+      }
+  }
+  __syncthreads();
+  const bool mirror = symmetric && tile.x != tile.y;
+  for (int e = threadIdx.x; e < nacc; e += blockDim.x) {
+    const double2 v = acc[e];
+    if (v.x == 0.0 && v.y == 0.0) continue;
+    const int lsi = e / ndt, lti = e - lsi * ndt;
+    const long long row = A.test_blk_dofs[t0 + lti], col = A.trial_blk_dofs[s0 + lsi];
+    store_value(A.out, row + A.ld * col, {v.x, v.y}, A.out_type, kAtomic);
+    if (mirror) store_value(A.out, col + A.ld * row, {v.x, v.y}, A.out_type, kAtomic);
+  }
+}
+
+// FP64 evaluation of the near (non-adjacent) cell pairs with a P1 side, one thread per
+// pair; the local block is added with atomics at the DOF indices.
+template <int Q, int OP, bool HELM, int CENTRED>
+__global__ void __launch_bounds__(256) near_pair_p1_kernel(const RegularArgs A, const int2* __restrict__ pairs,
+                                                           int npairs) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npairs) return;
+  const int2 pr = pairs[t];
+  const int i = pr.x, j = pr.y;
+  const double* tg = A.test_geo + (size_t)i * kGeoStride;
+  const double* sg = A.trial_geo + (size_t)j * kGeoStride;
+  double R2;
+  if (far_pair(tg + kGeoCentroid, sg + kGeoCentroid, tg[kGeoRadius], sg[kGeoRadius], &R2)) return;
+  double xp[Q][3], wp[Q], sp[Q * 4];
+#pragma unroll
+  for (int p = 0; p < Q; ++p) {
+    const double* s = A.test_pts + ((size_t)i * Q + p) * 4;
+    xp[p][0] = s[0];
+    xp[p][1] = s[1];
+    xp[p][2] = s[2];
+    wp[p] = s[3];
+  }
+#pragma unroll
+  for (int m = 0; m < Q * 4; ++m) sp[m] = A.trial_pts[(size_t)j * Q * 4 + m];
+  const double nx[3] = {tg[9], tg[10], tg[11]};
+  const double ct[3] = {tg[kGeoCentroid], tg[kGeoCentroid + 1], tg[kGeoCentroid + 2]};
+  cplx M[3][3], msum;
+  p1_local<Q, OP, HELM, CENTRED>(A, xp, wp, nx, ct, sp, sg, M, msum);
+#pragma unroll
+  for (int b = 0; b < 3; ++b)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (a >= A.nbt || b >= A.nbs) continue;
+      const cplx v = local_entry<OP, HELM>(M, a, b, msum, A.nbt, A.nbs, tg, sg, a, b, A.k);
+      const long long row = A.test_dofs[(size_t)i * A.nbt + a];
+      const long long col = A.trial_dofs[(size_t)j * A.nbs + b];
+      store_value(A.out, row + A.ld * col, v, A.out_type, kAtomic);
+    }
+}
+
 // FP64 evaluation of a list of near (non-adjacent) P0 x P0 cell pairs, one
 // thread per pair, added to their (unique) entries.
 template <int Q, int OP, bool HELM, bool CENTRED, typename Args>
@@ -357,4 +559,9 @@ __global__ void __launch_bounds__(256) batched_near_kernel(const RegularArgs* __
 }
 
 }  // namespace dev
+
+// p1mixed.hip
+void launch_mixed_lds(int op, bool helm, int tier, int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& A,
+                      const int2* tiles, int sym, size_t lds, const int2* near, int nnear);
+
 }  // namespace bempp_amd

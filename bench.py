@@ -176,11 +176,16 @@ def config_suite(steps, warmup):
     cases = [("laplace", "single_layer", "dp0", 3, 0.0, "fp64", np.float64),
              ("laplace", "double_layer", "p1", 5, 0.0, "fp64", np.float64),
              ("laplace", "hypersingular", "p1", 5, 0.0, "fp64", np.float64),
+             ("laplace", "double_layer", "p1", 5, 0.0, "mixed", np.float64),
+             ("laplace", "hypersingular", "p1", 5, 0.0, "mixed", np.float64),
              ("helmholtz", "single_layer", "dp0", 5, 5.0, "fp64", np.complex128),
              ("helmholtz", "single_layer", "dp0", 5, 5.0, "mixed", np.complex64),
              ("helmholtz", "double_layer", "p1", 5, 5.0, "fp64", np.complex128),
+             ("helmholtz", "double_layer", "p1", 5, 5.0, "mixed", np.complex64),
              ("helmholtz", "adjoint_double_layer", "p1", 5, 5.0, "fp64", np.complex128),
+             ("helmholtz", "adjoint_double_layer", "p1", 5, 5.0, "mixed", np.complex64),
              ("helmholtz", "hypersingular", "p1", 5, 5.0, "fp64", np.complex128),
+             ("helmholtz", "hypersingular", "p1", 5, 5.0, "mixed", np.complex64),
              ("laplace", "single_layer", "dp0", 6, 0.0, "fp64", np.float64)]
     grids = {}
     for kind, op, sp, level, k, prec, dt in cases:
@@ -208,6 +213,17 @@ def config_suite(steps, warmup):
             times.append(a.kernel_times())
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t0) * 1e3 / steps
+        fp64_diff = None
+        if prec == "mixed":  # whole-matrix difference to the FP64 GPU assembly (parity-tested against the oracle)
+            o64 = BoundaryAssemblerOptions()
+            o64.set_regular_quadrature_degree(ReferenceCellType.Triangle, 6)
+            o64.set_singular_quadrature_degree(ReferenceCellType.Triangle, ReferenceCellType.Triangle, 4)
+            a64 = (create_laplace_assembler(ops[op], o64) if kind == "laplace"
+                   else create_helmholtz_assembler(k, ops[op], o64))
+            ref64 = a64.assemble_device(space, space)
+            diff = (out.to(ref64.dtype) - ref64).norm() / ref64.norm()
+            fp64_diff = float(diff)
+            del ref64
         rows = min(32, n)
         err = None  # the oracle restricts rows for P0 test spaces only; P1 parity is in the GPU tests
         if sp == "dp0":
@@ -219,7 +235,7 @@ def config_suite(steps, warmup):
                 "entries_per_s": n * n / (ms * 1e-3),
                 "regular_kernels_ms": float(np.median([t[0] for t in times])),
                 "adjacent_kernel_ms": float(np.median([t[1] for t in times])),
-                "sample_rows": rows, "sample_rel_err_vs_oracle": err}
+                "sample_rows": rows, "sample_rel_err_vs_oracle": err, "rel_diff_vs_fp64_gpu": fp64_diff}
         print(json.dumps(line), flush=True)
         del out
         torch.cuda.empty_cache()

@@ -259,3 +259,31 @@ def test_mixed_precision_complex64_output(op):
     err = _rel(A.astype(np.complex128), R)
     print(f"[mixed-c64] {op}: rel Frobenius err {err:.3e}", flush=True)
     assert 1e-12 < err <= 1e-6
+
+
+@pytest.mark.parametrize("kind,k", [("laplace", 0.0), ("helmholtz", 2.5), ("helmholtz", 8.0)])
+@pytest.mark.parametrize("op", list(OPS))
+def test_mixed_precision_p1(kind, k, op):
+    """Mixed precision with P1 spaces (FP32 far field in the LDS-accumulating tiles, FP64 near
+    pairs with atomics, FP64 Duffy pairs), level-3 icosphere: <= 1e-6 against the FP64 oracle.
+    k = 8 exercises the range-reduced FP32 phase."""
+    g = icosphere(3)
+    s = _space(g, "p1")
+    a = _assembler(kind, op, k, _mixed_options())
+    A = a.assemble(s, s)
+    R = O.assemble(kind, op, g.points, g.cells, test=(1, True), trial=(1, True), k=k)
+    err = _rel(A, R)
+    print(f"[mixed-p1] {kind} {op} k={k}: rel Frobenius err {err:.3e}", flush=True)
+    assert err <= 1e-6
+    assert err > 1e-13  # the FP32 path ran
+
+
+@pytest.mark.parametrize("test,trial", [("dp0", "p1"), ("p1", "dp0")])
+def test_mixed_precision_mixed_spaces(test, trial):
+    g = icosphere(3)
+    a = _assembler("helmholtz", "double_layer", 2.5, _mixed_options())
+    A = a.assemble(_space(g, trial), _space(g, test))
+    R = O.assemble("helmholtz", "double_layer", g.points, g.cells, test=SPACES[test], trial=SPACES[trial], k=2.5)
+    err = _rel(A, R)
+    print(f"[mixed-p0p1] {test} x {trial}: rel Frobenius err {err:.3e}", flush=True)
+    assert 1e-13 < err <= 1e-6
