@@ -313,6 +313,61 @@ __device__ __forceinline__ void p1_local(Args& A, const double (&xp)[Q][3], cons
     }
 }
 
+// p1_local with the test points read per test point p (loop not unrolled) instead of held
+// in registers for the whole cell pair: the Helmholtz tile kernel's register footprint
+// drops from ~300 to ~165 VGPRs (the lane's points stay in the L1 cache across its trial
+// cells).
+template <int Q, int OP, bool HELM, int CENTRED, typename Args, typename SG>
+__device__ __forceinline__ void p1_local_streamed(Args& A, const double* tpts, const double (&nx)[3],
+                                                  const double (&ct)[3], const double (&sp)[Q * 4], SG sgeo,
+                                                  cplx (&M)[3][3], cplx& msum) {
+  const double ny[3] = {sgeo[9], sgeo[10], sgeo[11]};
+  double kR = 0.0;
+  cplx e = {1.0, 0.0};
+  if constexpr (CENTRED) {
+    const double cs[3] = {sgeo[kGeoCentroid], sgeo[kGeoCentroid + 1], sgeo[kGeoCentroid + 2]};
+    e = pair_phase(ct, cs, A.k, &kR);
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) M[a][b] = {0.0, 0.0};
+#pragma unroll 1
+  for (int p = 0; p < Q; ++p) {
+    const double x0 = tpts[4 * p], x1 = tpts[4 * p + 1], x2 = tpts[4 * p + 2], w = tpts[4 * p + 3];
+    cplx T[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const cplx kv = kernel_scalar<OP, HELM, CENTRED>(x0 - sp[4 * q], x1 - sp[4 * q + 1], x2 - sp[4 * q + 2], A.k,
+                                                       nx, ny, kR);
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        const double f = (A.nbs == 3 ? A.phi[q][b] : (b == 0 ? 1.0 : 0.0)) * sp[4 * q + 3];
+        T[b].re = fma(kv.re, f, T[b].re);
+        T[b].im = fma(kv.im, f, T[b].im);
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double f = (A.nbt == 3 ? A.phi[p][a] : (a == 0 ? 1.0 : 0.0)) * w;
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        M[a][b].re = fma(f, T[b].re, M[a][b].re);
+        M[a][b].im = fma(f, T[b].im, M[a][b].im);
+      }
+    }
+  }
+  msum = {0.0, 0.0};
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      if constexpr (CENTRED) M[a][b] = cmul(M[a][b], e);
+      msum.re += M[a][b].re;
+      msum.im += M[a][b].im;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Regular pairs with a P1 space on either side. One thread per test cell; the
 // block's trial cells are walked in order with wave-uniform (scalar) loads of
@@ -389,14 +444,18 @@ __global__ void __launch_bounds__(256) regular_tiled_lds(const RegularArgs A, co
   const int i = tile.x * kTile + lane;
   const bool i_ok = i < A.n_test;
   const int il = i_ok ? i : A.n_test - 1;
+  // Helmholtz: test points read per point (fewer registers, more waves); Laplace: held in
+  // registers (its kernel is short enough that the register version runs at higher occupancy)
+  const double* tpts = A.test_pts + (size_t)il * Q * 4;
   double xp[Q][3], wp[Q];
+  if constexpr (!HELM) {
 #pragma unroll
-  for (int p = 0; p < Q; ++p) {
-    const double* s = A.test_pts + ((size_t)il * Q + p) * 4;
-    xp[p][0] = s[0];
-    xp[p][1] = s[1];
-    xp[p][2] = s[2];
-    wp[p] = s[3];
+    for (int p = 0; p < Q; ++p) {
+      xp[p][0] = tpts[4 * p];
+      xp[p][1] = tpts[4 * p + 1];
+      xp[p][2] = tpts[4 * p + 2];
+      wp[p] = tpts[4 * p + 3];
+    }
   }
   const double* tgeo = A.test_geo + (size_t)il * kGeoStride;
   const double nx[3] = {tgeo[9], tgeo[10], tgeo[11]};
@@ -428,7 +487,10 @@ __global__ void __launch_bounds__(256) regular_tiled_lds(const RegularArgs A, co
                                           tv1 == sv2 || tv2 == sv0 || tv2 == sv1 || tv2 == sv2);
     if (adjacent || !i_ok) continue;
     cplx M[3][3], msum;
-    p1_local<Q, OP, HELM, CENTRED>(A, xp, wp, nx, ct, sp, sg, M, msum);
+    if constexpr (HELM)
+      p1_local_streamed<Q, OP, HELM, CENTRED>(A, tpts, nx, ct, sp, sg, M, msum);
+    else
+      p1_local<Q, OP, HELM, CENTRED>(A, xp, wp, nx, ct, sp, sg, M, msum);
 #pragma unroll
     for (int b = 0; b < 3; ++b)
 #pragma unroll
