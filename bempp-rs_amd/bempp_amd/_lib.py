@@ -120,6 +120,14 @@ def lib():
     return _lib
 
 
+def release(name, handle):
+    """Free a native handle with the C function `name`. At interpreter shutdown (module globals
+    already cleared) it does nothing: the process is ending and the HIP runtime may be gone."""
+    lib_ = _lib
+    if handle and lib_ is not None:
+        getattr(lib_, name)(handle)
+
+
 def last_error():
     return lib().bempp_last_error().decode()
 

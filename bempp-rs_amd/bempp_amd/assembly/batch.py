@@ -59,11 +59,9 @@ class BatchAssembler:
         oh = (ctypes.c_void_p * n)(*[o.data_ptr() for o in outputs])
         self._h = L.check_ptr(L.lib().boundary_assembler_batch_new(ah, sh, oh, n))
 
-    def __del__(self):
-        h = getattr(self, "_h", None)
-        if h:
-            L.lib().boundary_assembler_batch_t_free(h)
-            self._h = None
+    def __del__(self, _release=L.release):
+        _release("boundary_assembler_batch_t_free", getattr(self, "_h", None))
+        self._h = None
 
     def assemble(self, stream=None):
         """Enqueue the batch on `stream` (default: torch's current stream); returns the outputs."""

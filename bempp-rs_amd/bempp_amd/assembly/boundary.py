@@ -69,11 +69,9 @@ class BoundaryAssemblerOptions:
         self._read_only = False
         self._rs_options = L.check_ptr(L.lib().boundary_assembler_options_new())
 
-    def __del__(self):
-        h = getattr(self, "_rs_options", None)
-        if h:
-            L.lib().boundary_assembler_options_t_free(h)
-            self._rs_options = None
+    def __del__(self, _release=L.release):
+        _release("boundary_assembler_options_t_free", getattr(self, "_rs_options", None))
+        self._rs_options = None
 
     def set_regular_quadrature_degree(self, cell: ReferenceCellType, degree: int):
         """Set the (non-singular) quadrature degree (number of points) for a cell type."""
@@ -140,11 +138,10 @@ class BoundaryAssembler:
         self._rs_assembler = rs_assembler
         self._owned = owned
 
-    def __del__(self):
-        h = getattr(self, "_rs_assembler", None)
-        if h and self._owned:
-            L.lib().boundary_assembler_t_free(h)
-            self._rs_assembler = None
+    def __del__(self, _release=L.release):
+        if getattr(self, "_owned", False):
+            _release("boundary_assembler_t_free", getattr(self, "_rs_assembler", None))
+        self._rs_assembler = None
 
     @property
     def dtype(self):

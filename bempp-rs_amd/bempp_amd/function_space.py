@@ -13,11 +13,9 @@ class FunctionSpace:
         self._grid = grid  # keeps the grid alive
         self._family = family
 
-    def __del__(self):
-        h = getattr(self, "_rs_space", None)
-        if h:
-            L.lib().space_t_free(h)
-            self._rs_space = None
+    def __del__(self, _release=L.release):
+        _release("space_t_free", getattr(self, "_rs_space", None))
+        self._rs_space = None
 
     @property
     def grid(self):

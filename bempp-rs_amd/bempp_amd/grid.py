@@ -17,11 +17,9 @@ class Grid:
             _handle = L.check_ptr(L.lib().grid_new(L.ptr(points), len(points), L.ptr(cells), len(cells)))
         self._handle = _handle
 
-    def __del__(self):
-        h = getattr(self, "_handle", None)
-        if h:
-            L.lib().grid_t_free(h)
-            self._handle = None
+    def __del__(self, _release=L.release):
+        _release("grid_t_free", getattr(self, "_handle", None))
+        self._handle = None
 
     @property
     def topology_dim(self):

@@ -170,3 +170,27 @@ def test_batch_without_gpu_fails_cleanly():
     assert L.last_error()
     assert not L.lib().boundary_assembler_batch_new(ah, sh, oh, 0)
     assert "at least one" in L.last_error()
+
+
+def test_handles_held_by_module_globals_exit_cleanly():
+    """Objects still alive at interpreter shutdown are not freed through a half-torn-down module."""
+    import subprocess
+    import sys
+
+    pkg = Path(L.lib_path()).parent.parent
+    code = ("import torch\n"
+            "from bempp_amd.assembly.boundary import BoundaryAssemblerOptions, OperatorType, "
+            "create_helmholtz_assembler\n"
+            "from bempp_amd.element import Continuity, Family, create_family\n"
+            "from bempp_amd.function_space import function_space\n"
+            "from bempp_amd.shapes import icosphere\n"
+            "import bempp_amd._lib as L\n"
+            "g = icosphere(1)\n"
+            "s = function_space(g, create_family(Family.Lagrange, 0, Continuity.Discontinuous))\n"
+            "for k in [1.0, 2.0]:\n"
+            "    o = BoundaryAssemblerOptions()\n"
+            "    a = create_helmholtz_assembler(k, OperatorType.SingleLayer, o)\n")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120,
+                       env={**__import__("os").environ, "PYTHONPATH": str(pkg)})
+    assert r.returncode == 0, r.stderr
+    assert "Exception ignored" not in r.stderr, r.stderr
