@@ -627,6 +627,10 @@ Prepared prepare(const Assembler& a, const FunctionSpace& trial_c, const Functio
   P.nnear = 0;
   R.test_pts32 = R.trial_pts32 = nullptr;
   if (P.mixed) {
+    // the mixed P0 tiles sum point pairs per weight class (kernels_mixed.hpp)
+    for (int p = 0; p < P.Q; ++p)
+      if (w[p] != w[dev::class_point(P.Q, dev::weight_class(P.Q, p))])
+        throw Error("mixed precision: quadrature rule weights do not follow the orbit classes");
     R.test_pts32 = R.trial_pts32 = grid_points32_for_rule(tgrid, tc, P.Q, xi, w);
     P.near = grid_near_pairs(tgrid, tc, &P.nnear);
   }

@@ -22,6 +22,19 @@ namespace dev {
 
 constexpr double kNearFactor = 3.0;
 
+// Weight classes of the symmetric triangle rules (triangle_rule): the points of one orbit
+// share a weight, so far-field point-pair sums are taken per (test class, trial class) in
+// FP32, unweighted, and weighted once per cell pair in FP64. Q = 1, 3: one class;
+// Q = 6: {0,1,2} {3,4,5}; Q = 7: {0} {1,2,3} {4,5,6}. The host checks a rule against
+// these classes before launching a kernel that relies on them (rule_has_weight_classes).
+template <int Q>
+constexpr int kWeightClasses = Q == 6 ? 2 : Q == 7 ? 3 : 1;
+__host__ __device__ constexpr int weight_class(int Q, int p) {
+  return Q == 6 ? (p >= 3 ? 1 : 0) : Q == 7 ? (p == 0 ? 0 : p <= 3 ? 1 : 2) : 0;
+}
+// first point of class c
+__host__ __device__ constexpr int class_point(int Q, int c) { return Q == 7 ? (c == 0 ? 0 : 3 * c - 2) : 3 * c; }
+
 // Far-pair predicate shared by the mixed tile kernel and the near-pair kernel.
 // Written with explicit fma so both evaluate it bitwise identically: every
 // non-adjacent pair is taken by exactly one of them.
@@ -40,8 +53,8 @@ struct cplxf {
 
 // sin/cos in FP32 without reduction: TIER 1 for |x| <= 0.5, TIER 2 for |x| <= 0.25.
 // sin x = x + x^3 P(x^2), cos x = 1 + x^2 C(x^2) with weighted minimax P, C of degrees
-// 2/3 (tier 1) and 1/2 (tier 2) (Lawson's iteration at 50 digits); float32 Horner
-// errors below 1.8e-8 (sin) and 3.7e-8 (cos).
+// 2/3 (tier 1) and 1/1 (tier 2) (Lawson's iteration at 50 digits); float32 Horner
+// errors below 1.8e-8 (sin) and 3.7e-8 (cos) for tier 1, 7.8e-9 and 4.3e-8 for tier 2.
 template <int TIER>
 __device__ __forceinline__ void sincos_small_f32(float x, float* s, float* c) {
   static_assert(TIER == 1 || TIER == 2, "phase tiers: 1 (|x| <= 0.5), 2 (|x| <= 0.25)");
@@ -55,8 +68,7 @@ __device__ __forceinline__ void sincos_small_f32(float x, float* s, float* c) {
     q = fmaf(q, u, -0.5f);
   } else {
     p = fmaf(u, 8.3148913e-3f, -1.6666627e-1f);
-    q = fmaf(u, -1.3859142e-3f, 4.1666559e-2f);
-    q = fmaf(q, u, -0.5f);
+    q = fmaf(u, 4.15459625e-2f, -4.99997667e-1f);
   }
   *s = fmaf(x * u, p, x);
   *c = fmaf(u, q, 1.0f);
@@ -123,6 +135,36 @@ __device__ __forceinline__ void far_accumulate_f32(float s, float dn, float R2, 
   }
 }
 
+// Helmholtz far quadrature-point pair with lengths scaled by kappa = |k| (P0 tiles): given
+// sh = kappa^2 s / 2, R2s = (kappa R)^2 and Rs = kappa R, adds e^{ix} / r' (SL, HS) or
+// e^{ix} (1 - i r') dn / r'^3 (DL, ADL) to acc, with r' = kappa r and x = r' - Rs = kappa (r - R)
+// (the caller multiplies by kappa or kappa^3 and conjugates for k < 0). x starts as
+// r'^2 rsq(r'^2) - Rs (absolute error ~1e-7 r') and takes one Newton step on
+// x (2 Rs + x) = 2 sh, with the derivative 2 (Rs + x) = 2 r' taken from the same rsq, which
+// leaves a relative error of ~1e-7 in x without a division.
+template <int OP, int PH>
+__device__ __forceinline__ void far_accumulate_scaled(float sh, float dn, float R2s, float Rs, cplxf& acc) {
+  const float r2 = fmaf(2.0f, sh, R2s);
+  const float rinv = __builtin_amdgcn_rsqf(r2);
+  const float x0 = fmaf(r2, rinv, -Rs);
+  const float f = fmaf(x0, fmaf(0.5f, x0, Rs), -sh);
+  const float x = fmaf(-f, rinv, x0);
+  float sn, cs;
+  if constexpr (PH == 0)
+    sincos_f32_wide(x, &sn, &cs);
+  else
+    sincos_small_f32<PH>(x, &sn, &cs);
+  if constexpr (OP == 0 || OP == 3) {
+    acc.re = fmaf(cs, rinv, acc.re);
+    acc.im = fmaf(sn, rinv, acc.im);
+  } else {
+    const float rp = Rs + x;
+    const float f3 = dn * (rinv * rinv) * rinv;
+    acc.re = fmaf(fmaf(rp, sn, cs), f3, acc.re);
+    acc.im = fmaf(fmaf(-rp, cs, sn), f3, acc.im);
+  }
+}
+
 // Mixed-precision P0 x P0 tiles: same tiling, symmetry and LDS mirror as
 // p0_tile; near and adjacent pairs are written as zero. With offsets o_p, o_q of the
 // points from their cell centroids and D = c_test - c_trial,
@@ -132,22 +174,37 @@ __device__ __forceinline__ void far_accumulate_f32(float s, float dn, float R2, 
 template <int Q, int OP, bool HELM, int PH, int CH = kChunk, typename Args>
 __device__ __forceinline__ void p0_tile_mixed(Args& A, const int2 tile, int symmetric,
                                               double2 (&lds)[4][CH][kTile]) {
+  if constexpr (HELM) {
+    // k = 0: the Helmholtz kernels reduce to the Laplace ones (and the lengths cannot be scaled by k)
+    if (A.k == 0.0) {
+      p0_tile_mixed<Q, OP, false, PH, CH>(A, tile, symmetric, lds);
+      return;
+    }
+  }
   constexpr bool kDN = OP == 1 || OP == 2;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int i = tile.x * kTile + lane;
   const bool i_ok = i < A.n_test;
   const int il = i_ok ? i : A.n_test - 1;
-  float op[Q][3];
-  double wp[Q];
+  constexpr int NC = kWeightClasses<Q>;
+  // Helmholtz: lengths scaled by kappa = |k| and s carried as sigma s, sigma = kappa^2 / 2
+  // (far_accumulate_scaled); Laplace: unscaled, sigma = 1.
+  const double kappa = HELM ? fabs(A.k) : 1.0;
+  const float sig = HELM ? (float)(0.5 * kappa * kappa) : 1.0f;
+  // test point offsets o_p kept as m_p = -2 sigma o_p and sigma |o_p|^2
+  float mp[Q][3], oo[Q];
 #pragma unroll
   for (int p = 0; p < Q; ++p) {
     const float4 o = ((const float4*)A.test_pts32)[(size_t)il * Q + p];
-    op[p][0] = o.x;
-    op[p][1] = o.y;
-    op[p][2] = o.z;
-    wp[p] = A.test_pts[((size_t)il * Q + p) * 4 + 3];
+    mp[p][0] = -2.0f * sig * o.x;
+    mp[p][1] = -2.0f * sig * o.y;
+    mp[p][2] = -2.0f * sig * o.z;
+    oo[p] = sig * fmaf(o.x, o.x, fmaf(o.y, o.y, o.z * o.z));
   }
+  double wt[NC];  // test weight of each class (rule weight x Jacobian)
+#pragma unroll
+  for (int c = 0; c < NC; ++c) wt[c] = A.test_pts[((size_t)il * Q + class_point(Q, c)) * 4 + 3];
   const double* tgeo = A.test_geo + (size_t)il * kGeoStride;
   const float nx[3] = {(float)tgeo[9], (float)tgeo[10], (float)tgeo[11]};
   const double ct[3] = {tgeo[kGeoCentroid], tgeo[kGeoCentroid + 1], tgeo[kGeoCentroid + 2]};
@@ -159,8 +216,6 @@ __device__ __forceinline__ void p0_tile_mixed(Args& A, const int2 tile, int symm
   const float4* trial_pts32 = (const float4*)A.trial_pts32;
   const auto trial_geo = constant_view(A.trial_geo);
   const auto trial_vid = constant_view(A.trial_vid);
-  float kf;
-  asm volatile("v_mov_b32 %0, %1" : "=v"(kf) : "s"((float)A.k));
   static_assert(CH == 4 || CH == 8 || CH == 16, "chunk must divide 16 and 64");
   for (int c = 0; c < 16 / CH; ++c) {
     const int jbase = tile.y * kTile + wave * 16 + c * CH;
@@ -196,9 +251,12 @@ __device__ __forceinline__ void p0_tile_mixed(Args& A, const int2 tile, int symm
         const double R = R2 * rsqrt_nr(R2);
         const float D[3] = {(float)Dx, (float)Dy, (float)Dz};
         const float D2[3] = {2.0f * D[0], 2.0f * D[1], 2.0f * D[2]};
-        const float R2f = (float)R2, Rf = (float)R;
+        const float R2f = (float)(HELM ? kappa * kappa * R2 : R2), Rf = (float)(kappa * R);
         const float* nrm = OP == 1 ? ny : nx;
         const float sgn = OP == 2 ? -1.0f : 1.0f;
+        // o_p.n = -m_p.n / (2 sigma) (times the sign of the adjoint double layer)
+        const float hsc = -0.5f * sgn / sig;
+        const float hn[3] = {hsc * nrm[0], hsc * nrm[1], hsc * nrm[2]};
         // D.n in FP64: (x - y).n ~ r^2 / 2 on smooth surfaces, well below |D| |n|
         double Dn64 = 0.0;
         if constexpr (OP == 1) Dn64 = Dx * sg[9] + Dy * sg[10] + Dz * sg[11];
@@ -208,27 +266,47 @@ __device__ __forceinline__ void p0_tile_mixed(Args& A, const int2 tile, int symm
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
           const float ox = sp[4 * q], oy = sp[4 * q + 1], oz = sp[4 * q + 2];
-          bq[q] = fmaf(D2[0] - ox, ox, fmaf(D2[1] - oy, oy, (D2[2] - oz) * oz));
+          bq[q] = sig * fmaf(D2[0] - ox, ox, fmaf(D2[1] - oy, oy, (D2[2] - oz) * oz));
           if constexpr (kDN) nq[q] = sgn * fmaf(ox, nrm[0], fmaf(oy, nrm[1], oz * nrm[2]));
         }
+        cplxf cls[NC][NC];  // unweighted sums per (test class, trial class)
+#pragma unroll
+        for (int a = 0; a < NC; ++a)
+#pragma unroll
+          for (int b = 0; b < NC; ++b) cls[a][b] = {0.0f, 0.0f};
 #pragma unroll
         for (int p = 0; p < Q; ++p) {
-          const float ap = fmaf(D2[0] + op[p][0], op[p][0],
-                                fmaf(D2[1] + op[p][1], op[p][1], (D2[2] + op[p][2]) * op[p][2]));
-          const float mx = -2.0f * op[p][0], my = -2.0f * op[p][1], mz = -2.0f * op[p][2];
+          // sigma a_p = sigma o_p.(2D + o_p) = sigma |o_p|^2 - D.m_p
+          const float mx = mp[p][0], my = mp[p][1], mz = mp[p][2];
+          const float ap = fmaf(-D[0], mx, fmaf(-D[1], my, fmaf(-D[2], mz, oo[p])));
           float dnp = 0.0f;
-          if constexpr (kDN)
-            dnp = fmaf(sgn * op[p][0], nrm[0], fmaf(sgn * op[p][1], nrm[1], fmaf(sgn * op[p][2], nrm[2], Dn)));
-          cplxf apq = {0.0f, 0.0f};
+          if constexpr (kDN) dnp = fmaf(mx, hn[0], fmaf(my, hn[1], fmaf(mz, hn[2], Dn)));
 #pragma unroll
           for (int q = 0; q < Q; ++q) {
             const float s = fmaf(mx, sp[4 * q], fmaf(my, sp[4 * q + 1], fmaf(mz, sp[4 * q + 2], ap - bq[q])));
-            far_accumulate_f32<OP, HELM, PH>(s, kDN ? dnp - nq[q] : 0.0f, R2f, Rf, kf, sp[4 * q + 3], apq);
+            cplxf& cl = cls[weight_class(Q, p)][weight_class(Q, q)];
+            if constexpr (HELM)
+              far_accumulate_scaled<OP, PH>(s, kDN ? dnp - nq[q] : 0.0f, R2f, Rf, cl);
+            else
+              far_accumulate_f32<OP, false, PH>(s, kDN ? dnp - nq[q] : 0.0f, R2f, Rf, 0.0f, 1.0f, cl);
           }
-          acc.re = fma((double)apq.re, wp[p], acc.re);
-          acc.im = fma((double)apq.im, wp[p], acc.im);
+        }
+#pragma unroll
+        for (int a = 0; a < NC; ++a) {
+          cplx row = {0.0, 0.0};
+#pragma unroll
+          for (int b = 0; b < NC; ++b) {
+            const double ws = A.trial_pts[((size_t)j * Q + class_point(Q, b)) * 4 + 3];
+            row.re = fma((double)cls[a][b].re, ws, row.re);
+            row.im = fma((double)cls[a][b].im, ws, row.im);
+          }
+          acc.re = fma(row.re, wt[a], acc.re);
+          acc.im = fma(row.im, wt[a], acc.im);
         }
         if constexpr (HELM) {
+          // undo the length scaling (1/r = kappa / r'), conjugate for k < 0, rotate by e^{ikR}
+          const double ks = kDN ? kappa * kappa * kappa : kappa;
+          acc = {acc.re * ks, A.k < 0.0 ? -acc.im * ks : acc.im * ks};
           cplx e;
           sincos_cw(A.k * R, &e.im, &e.re);
           acc = cmul(acc, e);
@@ -259,8 +337,9 @@ __device__ __forceinline__ void p0_tile_mixed(Args& A, const int2 tile, int symm
   }
 }
 
+// 4 waves per SIMD (<= 128 VGPRs): FP32 dual issue needs several resident waves.
 template <int Q, int OP, bool HELM, int PH>
-__global__ void __launch_bounds__(256) regular_p0_tiled_mixed(const RegularArgs A, const int2* __restrict__ tiles,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) regular_p0_tiled_mixed(const RegularArgs A, const int2* __restrict__ tiles,
                                                               int symmetric) {
   __shared__ double2 lds[4][kChunk][kTile];
   p0_tile_mixed<Q, OP, HELM, PH>(A, tiles[blockIdx.x], symmetric, lds);

@@ -202,11 +202,14 @@ def _mixed_options(nq=6, ns=4):
     return o
 
 
-@pytest.mark.parametrize("kind,k", [("laplace", 0.0), ("helmholtz", 2.5)])
+@pytest.mark.parametrize("kind,k", [("laplace", 0.0), ("helmholtz", 2.5), ("helmholtz", 5.0), ("helmholtz", -2.5),
+                                    ("helmholtz", 0.0)])
 @pytest.mark.parametrize("op", list(OPS))
 def test_mixed_precision_p0(kind, k, op):
     """Mixed precision (FP32 far pairs, FP64 near and adjacent pairs) on the level-4 icosphere
-    (5120 cells; k = 2.5 satisfies the centred-phase bound): relative Frobenius error <= 1e-6
+    (5120 cells; |k| (rad_i + rad_j) <= 0.25 for k = 2.5, <= 0.5 for k = 5: both short phase
+    series; negative k; Helmholtz with k = 0, which the tiles evaluate with the Laplace
+    kernels): relative Frobenius error <= 1e-6
     against the FP64 oracle, and no entry dropped or counted twice (largest entrywise error
     relative to the largest entry <= 1e-5)."""
     g = icosphere(4)
