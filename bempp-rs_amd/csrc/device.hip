@@ -653,6 +653,12 @@ Prepared prepare(const Assembler& a, const FunctionSpace& trial_c, const Functio
     R.phi[p][1] = xi[2 * p];
     R.phi[p][2] = xi[2 * p + 1];
   }
+  for (int p = 0; p < dev::kMaxQ; ++p)
+    for (int b = 0; b < 3; ++b) {
+      const bool on = p < P.Q;
+      R.wphi_test[p][b] = on ? (float)(w[p] * (test.nb == 3 ? R.phi[p][b] : (b == 0 ? 1.0 : 0.0))) : 0.0f;
+      R.wphi_trial[p][b] = on ? (float)(w[p] * (trial.nb == 3 ? R.phi[p][b] : (b == 0 ? 1.0 : 0.0))) : 0.0f;
+    }
   R.out = out;
   R.ld = (long long)test.size;
   R.out_type = out_type(a.dtype);

@@ -61,6 +61,9 @@ struct RegularArgs {
   int trial_chunk;
   double k;
   double phi[kMaxQ][3];     // basis values at the regular points
+  // rule weight x basis value per side (a P0 side: {w, 0, 0}), FP32 (mixed-precision P1 tiles)
+  float wphi_test[kMaxQ][3];
+  float wphi_trial[kMaxQ][3];
   void* out;
   long long ld;
   int out_type;
@@ -419,6 +422,32 @@ __global__ void __launch_bounds__(256) regular_kernel(const RegularArgs A) {
   }
 }
 
+// Adds a tile's LDS block acc[trial DOF][test DOF] (block-local indices) to the output
+// with global atomics. A mirrored tile (symmetric operator, off-diagonal tile) also adds
+// the transpose, in a second pass whose consecutive threads take consecutive trial DOFs:
+// the transposed entries of one test DOF are then neighbours in memory, so a wave's
+// atomics fall into a few cache lines instead of one line per lane.
+template <typename Args>
+__device__ __forceinline__ void flush_tile(const Args& A, const double2* acc, int t0, int ndt, int s0, int nds,
+                                           bool mirror) {
+  const int nacc = ndt * nds;
+  for (int e = threadIdx.x; e < nacc; e += blockDim.x) {
+    const double2 v = acc[e];
+    if (v.x == 0.0 && v.y == 0.0) continue;
+    const int lsi = e / ndt, lti = e - lsi * ndt;
+    const long long row = A.test_blk_dofs[t0 + lti], col = A.trial_blk_dofs[s0 + lsi];
+    store_value(A.out, row + A.ld * col, {v.x, v.y}, A.out_type, kAtomic);
+  }
+  if (!mirror) return;
+  for (int e = threadIdx.x; e < nacc; e += blockDim.x) {
+    const int lti = e / nds, lsi = e - lti * nds;
+    const double2 v = acc[lsi * ndt + lti];
+    if (v.x == 0.0 && v.y == 0.0) continue;
+    const long long row = A.test_blk_dofs[t0 + lti], col = A.trial_blk_dofs[s0 + lsi];
+    store_value(A.out, col + A.ld * row, {v.x, v.y}, A.out_type, kAtomic);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Regular pairs with a P1 side, tiled: one workgroup (4 wavefronts) per tile of 64 test
 // x 64 trial cells (lane = test cell, 16 trial cells per wavefront). The local blocks of
@@ -503,15 +532,7 @@ __global__ void __launch_bounds__(256) regular_tiled_lds(const RegularArgs A, co
       }
   }
   __syncthreads();
-  const bool mirror = symmetric && tile.x != tile.y;
-  for (int e = threadIdx.x; e < nacc; e += blockDim.x) {
-    const double2 v = acc[e];
-    if (v.x == 0.0 && v.y == 0.0) continue;
-    const int lsi = e / ndt, lti = e - lsi * ndt;
-    const long long row = A.test_blk_dofs[t0 + lti], col = A.trial_blk_dofs[s0 + lsi];
-    store_value(A.out, row + A.ld * col, {v.x, v.y}, A.out_type, kAtomic);
-    if (mirror) store_value(A.out, col + A.ld * row, {v.x, v.y}, A.out_type, kAtomic);
-  }
+  flush_tile(A, acc, t0, ndt, s0, nds, symmetric && tile.x != tile.y);
 }
 
 // ---------------------------------------------------------------------------

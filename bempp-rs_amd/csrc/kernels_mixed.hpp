@@ -347,17 +347,58 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) r
 
 // ---------------------------------------------------------------------------
 // Mixed precision with a P1 side: the LDS-accumulating tiles of regular_tiled_lds with
-// far cell pairs evaluated in FP32 (same per-point-pair arithmetic as p0_tile_mixed;
-// the trial basis sums T_b(p) = sum_q w_q K_pq psi_b(q) in FP32, the test basis sums and
-// e^{ikR} in FP64). Near and adjacent pairs contribute nothing here: near pairs come from
-// near_pair_p1_kernel in FP64, adjacent pairs from the Duffy kernel.
+// far cell pairs evaluated in FP32, with the point-pair arithmetic of p0_tile_mixed. With
+// W_b(q) = rule weight x basis value (wphi_trial; a P0 side has {w, 0, 0}), the trial sums
+// T_b(p) = sum_q W_b(q) K_pq and the test sums M_ab = sum_p W_a(p) T_b(p) are taken in
+// FP32; the Jacobians, the length scaling and e^{ikR} are applied once per cell pair in
+// FP64. The Laplace hypersingular operator needs only sum_pq w_p w_q K_pq (its P1 surface
+// curls are constant on a cell). Near and adjacent pairs contribute nothing here: near
+// pairs come from near_pair_p1_kernel in FP64, adjacent pairs from the Duffy kernel.
 // ---------------------------------------------------------------------------
+
+// FP32 kernel value of a far point pair without weights, e^{ikR} and 1/(4 pi). Helmholtz:
+// lengths scaled by kappa as in far_accumulate_scaled (sh = kappa^2 s / 2); Laplace:
+// unscaled (sh = s, R2s = R^2).
+template <int OP, bool HELM, int PH>
+__device__ __forceinline__ cplxf far_value(float sh, float dn, float R2s, float Rs) {
+  if constexpr (HELM) {
+    const float r2 = fmaf(2.0f, sh, R2s);
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float x0 = fmaf(r2, rinv, -Rs);
+    const float f = fmaf(x0, fmaf(0.5f, x0, Rs), -sh);
+    const float x = fmaf(-f, rinv, x0);
+    float sn, cs;
+    if constexpr (PH == 0)
+      sincos_f32_wide(x, &sn, &cs);
+    else
+      sincos_small_f32<PH>(x, &sn, &cs);
+    if constexpr (OP == 0 || OP == 3) {
+      return {cs * rinv, sn * rinv};
+    } else {
+      const float rp = Rs + x;
+      const float f3 = dn * (rinv * rinv) * rinv;
+      return {fmaf(rp, sn, cs) * f3, fmaf(-rp, cs, sn) * f3};
+    }
+  } else {
+    const float rinv = __builtin_amdgcn_rsqf(R2s + sh);
+    if constexpr (OP == 0 || OP == 3)
+      return {rinv, 0.0f};
+    else
+      return {dn * (rinv * rinv) * rinv, 0.0f};
+  }
+}
+
 template <int Q, int OP, bool HELM, int PH>
-__global__ void __launch_bounds__(256) regular_tiled_lds_mixed(const RegularArgs A, const int2* __restrict__ tiles,
-                                                               int symmetric) {
+__device__ __forceinline__ void p1_tile_mixed(const RegularArgs& A, const int2 tile, int symmetric, double2* acc) {
+  if constexpr (HELM) {
+    // k = 0: the Helmholtz kernels reduce to the Laplace ones (and the lengths cannot be scaled by k)
+    if (A.k == 0.0) {
+      p1_tile_mixed<Q, OP, false, PH>(A, tile, symmetric, acc);
+      return;
+    }
+  }
   constexpr bool kDN = OP == 1 || OP == 2;
-  extern __shared__ double2 acc[];
-  const int2 tile = tiles[blockIdx.x];
+  constexpr bool kSum = OP == 3 && !HELM;
   const int t0 = A.test_blk_off[tile.x], ndt = A.test_blk_off[tile.x + 1] - t0;
   const int s0 = A.trial_blk_off[tile.y], nds = A.trial_blk_off[tile.y + 1] - s0;
   const int nacc = ndt * nds;
@@ -368,17 +409,23 @@ __global__ void __launch_bounds__(256) regular_tiled_lds_mixed(const RegularArgs
   const int i = tile.x * kTile + lane;
   const bool i_ok = i < A.n_test;
   const int il = i_ok ? i : A.n_test - 1;
-  float mp[Q][3], mq[Q];  // m = -2 o_p and |o_p|^2
-  double wp[Q];
+  const double kappa = HELM ? fabs(A.k) : 1.0;
+  const float sig = HELM ? (float)(0.5 * kappa * kappa) : 1.0f;
+  float mp[Q][3], mq[Q];  // -2 sigma o_p and sigma |o_p|^2
 #pragma unroll
   for (int p = 0; p < Q; ++p) {
     const float4 o = ((const float4*)A.test_pts32)[(size_t)il * Q + p];
-    mp[p][0] = -2.0f * o.x;
-    mp[p][1] = -2.0f * o.y;
-    mp[p][2] = -2.0f * o.z;
-    mq[p] = fmaf(o.x, o.x, fmaf(o.y, o.y, o.z * o.z));
-    wp[p] = A.test_pts[((size_t)il * Q + p) * 4 + 3];
+    mp[p][0] = -2.0f * sig * o.x;
+    mp[p][1] = -2.0f * sig * o.y;
+    mp[p][2] = -2.0f * sig * o.z;
+    mq[p] = sig * fmaf(o.x, o.x, fmaf(o.y, o.y, o.z * o.z));
   }
+  // trial weights x basis values in VGPRs: an SGPR operand would stop FP32 dual issue
+  float ws[Q][3];
+#pragma unroll
+  for (int q = 0; q < Q; ++q)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) asm volatile("v_mov_b32 %0, %1" : "=v"(ws[q][b]) : "s"(A.wphi_trial[q][b]));
   const double* tgeo = A.test_geo + (size_t)il * kGeoStride;
   const float nx[3] = {(float)tgeo[9], (float)tgeo[10], (float)tgeo[11]};
   const double ct[3] = {tgeo[kGeoCentroid], tgeo[kGeoCentroid + 1], tgeo[kGeoCentroid + 2]};
@@ -392,20 +439,17 @@ __global__ void __launch_bounds__(256) regular_tiled_lds_mixed(const RegularArgs
   const auto trial_geo = constant_view(A.trial_geo);
   const auto trial_vid = constant_view(A.trial_vid);
   const auto trial_ldof = constant_view(A.trial_ldof);
-  float kf;
-  asm volatile("v_mov_b32 %0, %1" : "=v"(kf) : "s"((float)A.k));
 #pragma unroll 1
   for (int jj = 0; jj < 16; ++jj) {
     const int j = tile.y * kTile + wave * 16 + jj;
     if (j >= A.n_trial) break;  // wave-uniform
-    float sp[Q * 4];
+    float sp[Q][3];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const float4 t = trial_pts32[(size_t)j * Q + q];
-      sp[4 * q] = t.x;
-      sp[4 * q + 1] = t.y;
-      sp[4 * q + 2] = t.z;
-      sp[4 * q + 3] = t.w;
+      sp[q][0] = t.x;
+      sp[q][1] = t.y;
+      sp[q][2] = t.z;
     }
     const auto sg = trial_geo + (size_t)j * kGeoStride;
     const double cs[3] = {sg[kGeoCentroid], sg[kGeoCentroid + 1], sg[kGeoCentroid + 2]};
@@ -425,86 +469,101 @@ __global__ void __launch_bounds__(256) regular_tiled_lds_mixed(const RegularArgs
     const double R = R2 * rsqrt_nr(R2);
     const float D[3] = {(float)Dx, (float)Dy, (float)Dz};
     const float D2[3] = {2.0f * D[0], 2.0f * D[1], 2.0f * D[2]};
-    const float R2f = (float)R2, Rf = (float)R;
+    const float R2s = (float)(HELM ? kappa * kappa * R2 : R2), Rs = (float)(kappa * R);
     const float* nrm = OP == 1 ? ny : nx;
     const float sgn = OP == 2 ? -1.0f : 1.0f;
     double Dn64 = 0.0;
     if constexpr (OP == 1) Dn64 = Dx * sg[9] + Dy * sg[10] + Dz * sg[11];
     if constexpr (OP == 2) Dn64 = -(Dx * tgeo[9] + Dy * tgeo[10] + Dz * tgeo[11]);
     const float Dn = (float)Dn64;
-    const float nh[3] = {-0.5f * sgn * nrm[0], -0.5f * sgn * nrm[1], -0.5f * sgn * nrm[2]};
-    float bq[Q], nq[Q], ph[Q][3];
+    // o_p.n = -m_p.n / (2 sigma) (times the sign of the adjoint double layer)
+    const float hsc = -0.5f * sgn / sig;
+    const float hn[3] = {hsc * nrm[0], hsc * nrm[1], hsc * nrm[2]};
+    float bq[Q], nq[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      const float ox = sp[4 * q], oy = sp[4 * q + 1], oz = sp[4 * q + 2];
-      bq[q] = fmaf(D2[0] - ox, ox, fmaf(D2[1] - oy, oy, (D2[2] - oz) * oz));
+      const float ox = sp[q][0], oy = sp[q][1], oz = sp[q][2];
+      bq[q] = sig * fmaf(D2[0] - ox, ox, fmaf(D2[1] - oy, oy, (D2[2] - oz) * oz));
       nq[q] = kDN ? sgn * fmaf(ox, nrm[0], fmaf(oy, nrm[1], oz * nrm[2])) : 0.0f;
-#pragma unroll
-      for (int b = 0; b < 3; ++b) ph[q][b] = A.nbs == 3 ? (float)A.phi[q][b] : (b == 0 ? 1.0f : 0.0f);
     }
-    cplx M[3][3];
+    cplxf M[3][3];
+    float S = 0.0f;
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
-      for (int b = 0; b < 3; ++b) M[a][b] = {0.0, 0.0};
+      for (int b = 0; b < 3; ++b) M[a][b] = {0.0f, 0.0f};
 #pragma unroll
     for (int p = 0; p < Q; ++p) {
       const float ap = fmaf(-mp[p][0], D[0], fmaf(-mp[p][1], D[1], fmaf(-mp[p][2], D[2], mq[p])));
-      const float dnp = kDN ? fmaf(mp[p][0], nh[0], fmaf(mp[p][1], nh[1], fmaf(mp[p][2], nh[2], Dn))) : 0.0f;
+      const float dnp = kDN ? fmaf(mp[p][0], hn[0], fmaf(mp[p][1], hn[1], fmaf(mp[p][2], hn[2], Dn))) : 0.0f;
       cplxf T[3] = {{0.0f, 0.0f}, {0.0f, 0.0f}, {0.0f, 0.0f}};
+      float T0 = 0.0f;
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
-        const float s = fmaf(mp[p][0], sp[4 * q], fmaf(mp[p][1], sp[4 * q + 1], fmaf(mp[p][2], sp[4 * q + 2], ap - bq[q])));
-        cplxf kw = {0.0f, 0.0f};
-        far_accumulate_f32<OP, HELM, PH>(s, dnp - nq[q], R2f, Rf, kf, sp[4 * q + 3], kw);
+        const float s = fmaf(mp[p][0], sp[q][0], fmaf(mp[p][1], sp[q][1], fmaf(mp[p][2], sp[q][2], ap - bq[q])));
+        const cplxf kv = far_value<OP, HELM, PH>(s, dnp - nq[q], R2s, Rs);
+        if constexpr (kSum) {
+          T0 = fmaf(kv.re, ws[q][0] + ws[q][1] + ws[q][2], T0);
+        } else {
 #pragma unroll
-        for (int b = 0; b < 3; ++b) {
-          T[b].re = fmaf(kw.re, ph[q][b], T[b].re);
-          T[b].im = fmaf(kw.im, ph[q][b], T[b].im);
+          for (int b = 0; b < 3; ++b) {
+            T[b].re = fmaf(kv.re, ws[q][b], T[b].re);
+            if constexpr (HELM) T[b].im = fmaf(kv.im, ws[q][b], T[b].im);
+          }
         }
       }
+      if constexpr (kSum) {
+        S = fmaf(A.wphi_test[p][0] + A.wphi_test[p][1] + A.wphi_test[p][2], T0, S);
+      } else {
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const double f = (A.nbt == 3 ? A.phi[p][a] : (a == 0 ? 1.0 : 0.0)) * wp[p];
+        for (int a = 0; a < 3; ++a)
 #pragma unroll
-        for (int b = 0; b < 3; ++b) {
-          M[a][b].re = fma(f, (double)T[b].re, M[a][b].re);
-          M[a][b].im = fma(f, (double)T[b].im, M[a][b].im);
-        }
+          for (int b = 0; b < 3; ++b) {
+            M[a][b].re = fmaf(A.wphi_test[p][a], T[b].re, M[a][b].re);
+            if constexpr (HELM) M[a][b].im = fmaf(A.wphi_test[p][a], T[b].im, M[a][b].im);
+          }
       }
     }
-    cplx e = {1.0, 0.0};
-    if constexpr (HELM) sincos_cw(A.k * R, &e.im, &e.re);
+    // FP64: Jacobians, the length scaling (1/r = kappa / r'), k < 0 (conjugate), e^{ikR}
+    double scale = tgeo[12] * sg[12];
+    if constexpr (HELM) scale *= kDN ? kappa * kappa * kappa : kappa;
+    cplx e = {scale, 0.0};
+    if constexpr (HELM) {
+      sincos_cw(A.k * R, &e.im, &e.re);
+      e = {e.re * scale, e.im * scale};
+    }
+    const double sk = HELM && A.k < 0.0 ? -1.0 : 1.0;
+    cplx M64[3][3];
     cplx msum = {0.0, 0.0};
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
       for (int b = 0; b < 3; ++b) {
-        if constexpr (HELM) M[a][b] = cmul(M[a][b], e);
-        msum.re += M[a][b].re;
-        msum.im += M[a][b].im;
+        M64[a][b] = cmul({(double)M[a][b].re, sk * (double)M[a][b].im}, e);
+        msum.re += M64[a][b].re;
+        msum.im += M64[a][b].im;
       }
+    if constexpr (kSum) msum = {(double)S * scale, 0.0};
 #pragma unroll
     for (int b = 0; b < 3; ++b)
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
         if (a >= A.nbt || b >= A.nbs) continue;
-        const cplx v = local_entry<OP, HELM>(M, a, b, msum, A.nbt, A.nbs, tgeo, sg, a, b, A.k);
+        const cplx v = local_entry<OP, HELM>(M64, a, b, msum, A.nbt, A.nbs, tgeo, sg, a, b, A.k);
         double2* slot = acc + (ls[b] * ndt + lt[a]);
         atomicAdd(&slot->x, v.re);
         if constexpr (HELM) atomicAdd(&slot->y, v.im);
       }
   }
   __syncthreads();
-  const bool mirror = symmetric && tile.x != tile.y;
-  for (int e = threadIdx.x; e < nacc; e += blockDim.x) {
-    const double2 v = acc[e];
-    if (v.x == 0.0 && v.y == 0.0) continue;
-    const int lsi = e / ndt, lti = e - lsi * ndt;
-    const long long row = A.test_blk_dofs[t0 + lti], col = A.trial_blk_dofs[s0 + lsi];
-    store_value(A.out, row + A.ld * col, {v.x, v.y}, A.out_type, kAtomic);
-    if (mirror) store_value(A.out, col + A.ld * row, {v.x, v.y}, A.out_type, kAtomic);
-  }
+  flush_tile(A, acc, t0, ndt, s0, nds, symmetric && tile.x != tile.y);
+}
+
+template <int Q, int OP, bool HELM, int PH>
+__global__ void __launch_bounds__(256) regular_tiled_lds_mixed(const RegularArgs A, const int2* __restrict__ tiles,
+                                                               int symmetric) {
+  extern __shared__ double2 acc[];
+  p1_tile_mixed<Q, OP, HELM, PH>(A, tiles[blockIdx.x], symmetric, acc);
 }
 
 // FP64 evaluation of the near (non-adjacent) cell pairs with a P1 side, one thread per
