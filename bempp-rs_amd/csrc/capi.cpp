@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <complex>
 #include <cstring>
+#include <memory>
 #include <numeric>
 
 #include "../../include/bempp_amd.h"
@@ -21,9 +22,19 @@ struct BoundaryAssemblerT {
 };
 struct GridT {
   Grid* g;
+  // topology vertex index -> point index (grids built from ndgrid data; empty: identity)
+  std::vector<int64_t> topo_vertex_point;
 };
 struct SpaceT {
-  FunctionSpace* s;
+  std::shared_ptr<FunctionSpace> s;
+  const GridT* grid = nullptr;
+  uint8_t dtype = BEMPP_DTYPE_F64;
+  mutable std::vector<int64_t> point_dof;  // P1 continuous: point -> DOF (built on first use)
+};
+struct ElementFamilyT {
+  size_t degree;
+  uint8_t continuity;
+  uint8_t dtype;
 };
 struct MatrixT {
   bool csr = false;
@@ -184,21 +195,162 @@ int grid_set_points(GridT* g, const double* p) {
       1);
 }
 
+// Grid from the arrays of an ndgrid single-element grid (src/bindings/ndgrid.rs:16-56).
+// Flat triangles only (tdim 2, gdim 3, geometry degree 1, 3 points per cell). The arrays are
+// copied. The topology arrays are optional; when given, the cells' topology vertices must be a
+// relabelling of their geometry points with the same local order, and the relabelling is kept
+// for entity-indexed queries (space_get_local_dof_numbers with entity_dim 0).
+GridT* single_element_grid_borrowed_create_bempp(size_t tdim, const size_t*, const size_t* const*,
+                                                 const uint8_t* entity_types, const size_t* entity_counts,
+                                                 const size_t* const* const* downward_connectivity,
+                                                 const size_t* const* downward_connectivity_shape0,
+                                                 const size_t* const* const* const*, const size_t* const* const*,
+                                                 const void* points, size_t gdim, size_t npoints, uint8_t dtype,
+                                                 const size_t* cells, size_t points_per_cell, size_t ncells,
+                                                 size_t geometry_degree) {
+  return guarded(
+      [&]() -> GridT* {
+        need(points, "points");
+        need(cells, "cells");
+        if (tdim != 2) throw Error("only grids of topological dimension 2 are supported, got " + std::to_string(tdim));
+        if (gdim != 3) throw Error("only grids in 3 dimensions are supported, got " + std::to_string(gdim));
+        if (geometry_degree != 1 || points_per_cell != 3)
+          throw Error("only flat triangles (geometry degree 1, 3 points per cell) are supported");
+        if (entity_types && entity_types[2] != BEMPP_CELL_TRIANGLE)
+          throw Error("only triangle cells are supported, got cell type " + std::to_string(entity_types[2]));
+        if (dtype != BEMPP_DTYPE_F32 && dtype != BEMPP_DTYPE_F64)
+          throw Error("grid points must be f32 or f64, got dtype " + std::to_string(dtype));
+        if (entity_counts && entity_counts[2] != ncells)
+          throw Error("entity_counts[2] does not match the number of cells");
+        std::vector<double> p(3 * npoints);
+        for (size_t i = 0; i < 3 * npoints; ++i)
+          p[i] = dtype == BEMPP_DTYPE_F64 ? ((const double*)points)[i] : (double)((const float*)points)[i];
+        std::vector<int64_t> c(3 * ncells);
+        for (size_t i = 0; i < 3 * ncells; ++i) {
+          if (cells[i] >= npoints) throw Error("cell " + std::to_string(i / 3) + " refers to a missing point");
+          c[i] = (int64_t)cells[i];
+        }
+        std::vector<int64_t> topo;
+        const size_t* tc = downward_connectivity && downward_connectivity[2] ? downward_connectivity[2][0] : nullptr;
+        if (tc) {
+          if (downward_connectivity_shape0 && downward_connectivity_shape0[2] && downward_connectivity_shape0[2][0] != 3)
+            throw Error("cell-to-vertex connectivity must have 3 vertices per cell");
+          const size_t nvert = entity_counts ? entity_counts[0] : npoints;
+          topo.assign(nvert, -1);
+          std::vector<int64_t> point_topo(npoints, -1);
+          for (size_t i = 0; i < 3 * ncells; ++i) {
+            const size_t v = tc[i];
+            if (v >= nvert) throw Error("cell " + std::to_string(i / 3) + " refers to a missing vertex");
+            if ((topo[v] >= 0 && topo[v] != c[i]) || (point_topo[c[i]] >= 0 && point_topo[c[i]] != (int64_t)v))
+              throw Error("topology and geometry cells differ by more than a relabelling of the vertices");
+            topo[v] = c[i];
+            point_topo[c[i]] = (int64_t)v;
+          }
+        }
+        auto* g = new GridT{new Grid(std::move(p), std::move(c)), std::move(topo)};
+        return g;
+      },
+      (GridT*)nullptr);
+}
+void grid_t_free_bempp(GridT* g) { grid_t_free(g); }
+
+// ---- element families (src/bindings/ndelement.rs:8-44) ----
+ElementFamilyT* create_lagrange_family_bempp(size_t degree, uint8_t continuity, uint8_t dtype) {
+  return guarded(
+      [&]() -> ElementFamilyT* {
+        if (continuity > 1) throw Error("continuity must be 0 (standard) or 1 (discontinuous)");
+        if (dtype > BEMPP_DTYPE_C64) throw Error("invalid dtype " + std::to_string(dtype));
+        return new ElementFamilyT{degree, continuity, dtype};
+      },
+      (ElementFamilyT*)nullptr);
+}
+ElementFamilyT* create_raviart_thomas_family_bempp(size_t, uint8_t, uint8_t) {
+  set_last_error("Raviart-Thomas spaces are not supported (no operator of this library uses them)");
+  return nullptr;
+}
+ElementFamilyT* create_nedelec_family_bempp(size_t, uint8_t, uint8_t) {
+  set_last_error("Nedelec spaces are not supported (no operator of this library uses them)");
+  return nullptr;
+}
+void element_family_t_free_bempp(ElementFamilyT* f) { delete f; }
+
 // ---- spaces ----
 SpaceT* function_space_new(const GridT* g, size_t degree, uint8_t continuity) {
   return guarded(
       [&]() -> SpaceT* {
         need(g, "grid");
         if (continuity > 1) throw Error("continuity must be 0 (standard) or 1 (discontinuous)");
-        return new SpaceT{new FunctionSpace(g->g, (int)degree, continuity == BEMPP_CONTINUITY_STANDARD)};
+        auto* sp = new SpaceT{std::make_shared<FunctionSpace>(g->g, (int)degree, continuity == BEMPP_CONTINUITY_STANDARD)};
+        sp->grid = g;
+        return sp;
       },
       (SpaceT*)nullptr);
 }
-void space_t_free(SpaceT* s) {
-  if (!s) return;
-  delete s->s;
-  delete s;
+SpaceT* function_space(const GridT* g, const ElementFamilyT* family) {
+  return guarded(
+      [&]() -> SpaceT* {
+        need(family, "element family");
+        SpaceT* sp = function_space_new(g, family->degree, family->continuity);
+        if (!sp) throw Error(last_error());
+        sp->dtype = family->dtype;
+        return sp;
+      },
+      (SpaceT*)nullptr);
 }
+SpaceT* space_local_space(const SpaceT* s) {
+  return guarded(
+      [&]() -> SpaceT* {
+        need(s, "space");
+        return new SpaceT{s->s, s->grid, s->dtype, {}};
+      },
+      (SpaceT*)nullptr);
+}
+void space_t_free(SpaceT* s) { delete s; }
+uint8_t space_dtype(const SpaceT* s) { return s ? s->dtype : BEMPP_DTYPE_F64; }
+
+namespace {
+// DOFs of an entity (src/function.rs:258-260 over assign_dofs, :365-445): a continuous P1
+// space has one DOF per vertex; P0 and discontinuous P1 spaces have theirs on the cell.
+const int64_t* entity_dofs(const SpaceT* s, size_t dim, size_t index, size_t* n) {
+  *n = 0;
+  if (!s) return nullptr;
+  const FunctionSpace& fs = *s->s;
+  if (fs.continuous && fs.degree == 1) {
+    if (dim != 0) return nullptr;
+    int64_t point = (int64_t)index;
+    if (s->grid && !s->grid->topo_vertex_point.empty()) {
+      if (index >= s->grid->topo_vertex_point.size()) return nullptr;
+      point = s->grid->topo_vertex_point[index];
+    }
+    if (s->point_dof.empty()) {
+      s->point_dof.assign(fs.grid->nv(), -1);
+      for (size_t i = 0; i < fs.dofs.size(); ++i) s->point_dof[fs.grid->cells[i]] = fs.dofs[i];
+    }
+    if (point < 0 || point >= (int64_t)s->point_dof.size() || s->point_dof[point] < 0) return nullptr;
+    *n = 1;
+    return &s->point_dof[point];
+  }
+  if (dim != 2 || index >= (size_t)fs.grid->nc()) return nullptr;
+  *n = (size_t)fs.nb;
+  return &fs.dofs[index * fs.nb];
+}
+}  // namespace
+
+size_t space_get_local_dof_numbers_size(const SpaceT* s, size_t entity_dim, size_t entity_index) {
+  size_t n;
+  entity_dofs(s, entity_dim, entity_index, &n);
+  return n;
+}
+void space_get_local_dof_numbers(const SpaceT* s, size_t entity_dim, size_t entity_index, uintptr_t* dofs) {
+  size_t n;
+  const int64_t* d = entity_dofs(s, entity_dim, entity_index, &n);
+  if (dofs)
+    for (size_t i = 0; i < n; ++i) dofs[i] = (uintptr_t)d[i];
+}
+// Serial spaces own every DOF (src/function.rs:355-357); ghost queries get rank 0 and the index.
+int space_is_owned(const SpaceT* s, size_t local) { return s && local < (size_t)s->s->size ? 1 : 0; }
+size_t space_ownership_process(const SpaceT*, size_t) { return 0; }
+size_t space_ownership_index(const SpaceT*, size_t local) { return local; }
 size_t space_global_size(const SpaceT* s) { return s ? (size_t)s->s->size : 0; }
 size_t space_local_size(const SpaceT* s) { return s ? (size_t)s->s->size : 0; }
 int space_is_serial(const SpaceT* s) { return s ? 1 : 0; }
@@ -455,7 +607,7 @@ BoundaryAssemblerBatchT* boundary_assembler_batch_new(const BoundaryAssemblerT* 
           need(assemblers[i], "assembler");
           need(spaces[i], "space");
           a[i] = assemblers[i]->a;
-          s[i] = spaces[i]->s;
+          s[i] = spaces[i]->s.get();
         }
         Batch* b = batch_new(a, s, o);
         return new BoundaryAssemblerBatchT{b};

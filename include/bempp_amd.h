@@ -68,6 +68,7 @@ typedef struct BoundaryAssemblerT BoundaryAssemblerT;
 typedef struct GridT GridT;
 typedef struct SpaceT SpaceT;
 typedef struct MatrixT MatrixT;
+typedef struct ElementFamilyT ElementFamilyT;
 
 /* Last error message of the calling thread (no counterpart: the reference panics). */
 const char* bempp_last_error(void);
@@ -127,10 +128,57 @@ void grid_cells_copy(const GridT* grid, int64_t* cells);
 /* Replace the vertex coordinates (same topology); drops cached device geometry. */
 int grid_set_points(GridT* grid, const double* points);
 
-/* replaces function_space(grid, family) (src/bindings/space.rs:20-42) with the
- * Lagrange family given by degree (0 or 1) and continuity. */
+/* The entry points the reference's python layer binds for grids, element families and
+ * spaces (python/bempp/function_space.py:55-64,143-191), under the same names. */
+/* replaces single_element_grid_borrowed_create_bempp (src/bindings/ndgrid.rs:16-56): a grid
+ * from the arrays of an ndgrid single-element grid. Flat triangles only (tdim 2, gdim 3,
+ * geometry degree 1, 3 points per cell, entity_types[2] triangle); points f32 or f64
+ * [npoints][3], cells [ncells][3] point indices. Identifier and upward-connectivity arrays
+ * are not used; the cell-to-vertex topology (downward_connectivity[2][0], optional) must be a
+ * relabelling of the cells' points. The arrays are copied. NULL on error. */
+GridT* single_element_grid_borrowed_create_bempp(size_t tdim, const size_t* id_sizes, const size_t* const* id_pointers,
+                                                 const uint8_t* entity_types, const size_t* entity_counts,
+                                                 const size_t* const* const* downward_connectivity,
+                                                 const size_t* const* downward_connectivity_shape0,
+                                                 const size_t* const* const* const* upward_connectivity,
+                                                 const size_t* const* const* upward_connectivity_lens,
+                                                 const void* points, size_t gdim, size_t npoints, uint8_t dtype,
+                                                 const size_t* cells, size_t points_per_cell, size_t ncells,
+                                                 size_t geometry_degree);
+/* replaces grid_t_free_bempp (src/bindings/ndgrid.rs:6-14) */
+void grid_t_free_bempp(GridT* grid);
+/* replaces create_lagrange_family_bempp (src/bindings/ndelement.rs:18-25); degree 0
+ * (discontinuous) or 1 are the spaces function_space accepts. */
+ElementFamilyT* create_lagrange_family_bempp(size_t degree, uint8_t continuity, uint8_t dtype);
+/* replace create_raviart_thomas_family_bempp / create_nedelec_family_bempp
+ * (src/bindings/ndelement.rs:27-44): vector-valued spaces are not supported; NULL. */
+ElementFamilyT* create_raviart_thomas_family_bempp(size_t degree, uint8_t continuity, uint8_t dtype);
+ElementFamilyT* create_nedelec_family_bempp(size_t degree, uint8_t continuity, uint8_t dtype);
+/* replaces element_family_t_free_bempp (src/bindings/ndelement.rs:8-16) */
+void element_family_t_free_bempp(ElementFamilyT* family);
+/* replaces function_space(grid, family) (src/bindings/space.rs:20-42); the space borrows
+ * the grid (free the grid after its spaces). */
+SpaceT* function_space(const GridT* grid, const ElementFamilyT* family);
+/* function_space with the Lagrange family given by degree (0 or 1) and continuity. */
 SpaceT* function_space_new(const GridT* grid, size_t degree, uint8_t continuity);
+/* replaces space_local_space (src/bindings/space.rs:44-56): a serial space is its own
+ * local space (a new handle to the same space). */
+SpaceT* space_local_space(const SpaceT* space);
 void space_t_free(SpaceT* space);
+/* replaces space_get_local_dof_numbers_size / space_get_local_dof_numbers
+ * (src/bindings/space.rs:69-107): DOFs of entity (entity_dim, entity_index) -- a vertex of a
+ * continuous P1 space, a cell of a P0 or discontinuous P1 space; none otherwise. */
+size_t space_get_local_dof_numbers_size(const SpaceT* space, size_t entity_dim, size_t entity_index);
+void space_get_local_dof_numbers(const SpaceT* space, size_t entity_dim, size_t entity_index, uintptr_t* dofs);
+/* replaces space_is_owned (src/bindings/space.rs:178-187): a serial space owns every DOF.
+ * space_ownership_process / space_ownership_index are what python/bempp/function_space.py:96-103
+ * calls for a ghost DOF (the reference's Rust bindings do not define them): rank 0, the index. */
+int space_is_owned(const SpaceT* space, size_t local_dof_index);
+size_t space_ownership_process(const SpaceT* space, size_t local_dof_index);
+size_t space_ownership_index(const SpaceT* space, size_t local_dof_index);
+/* replaces space_dtype (src/bindings/space.rs:189-198): the element family's dtype (F64 for
+ * function_space_new). */
+uint8_t space_dtype(const SpaceT* space);
 /* Space accessors (python/bempp/function_space.py:256-311) */
 size_t space_global_size(const SpaceT* space);
 size_t space_local_size(const SpaceT* space);

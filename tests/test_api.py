@@ -194,3 +194,140 @@ def test_handles_held_by_module_globals_exit_cleanly():
                        env={**__import__("os").environ, "PYTHONPATH": str(pkg)})
     assert r.returncode == 0, r.stderr
     assert "Exception ignored" not in r.stderr, r.stderr
+
+
+def _ndgrid_args(points, cells, perm, dtype=np.float64, tdim=2, cell_type=2, topo=True):
+    """The argument list of single_element_grid_borrowed_create_bempp for a triangle grid whose
+    topology numbers vertex p as perm[p] (python/bempp/function_space.py:155-178 passes ndgrid's
+    arrays the same way)."""
+    sz = ctypes.c_size_t
+    keep = []
+    pts = np.ascontiguousarray(points, dtype=dtype)
+    cls = np.ascontiguousarray(cells, dtype=np.uintp)
+    tcells = np.ascontiguousarray(perm[cells], dtype=np.uintp)
+    etypes = np.array([0, 1, cell_type], dtype=np.uint8)
+    counts = np.array([len(points), 0, len(cells)], dtype=np.uintp)
+    keep += [pts, cls, tcells, etypes, counts]
+    row2 = (ctypes.c_void_p * 3)(tcells.ctypes.data if topo else None, None, None)
+    down = (ctypes.c_void_p * 3)(None, None, ctypes.addressof(row2))
+    shape2 = np.array([3, 0, 0], dtype=np.uintp)
+    shp = (ctypes.c_void_p * 3)(None, None, shape2.ctypes.data)
+    keep += [row2, down, shape2, shp]
+    args = [sz(tdim), None, None, etypes.ctypes.data_as(ctypes.c_void_p), counts.ctypes.data_as(ctypes.c_void_p),
+            ctypes.cast(down, ctypes.c_void_p), ctypes.cast(shp, ctypes.c_void_p), None, None,
+            pts.ctypes.data_as(ctypes.c_void_p), sz(3), sz(len(points)),
+            ctypes.c_uint8(0 if dtype == np.float32 else 1), cls.ctypes.data_as(ctypes.c_void_p), sz(3),
+            sz(len(cells)), sz(1)]
+    return args, keep
+
+
+def _raw_lib():
+    lib = ctypes.CDLL(L.lib_path())
+    vp, sz, u8 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint8
+    lib.single_element_grid_borrowed_create_bempp.restype = vp
+    lib.single_element_grid_borrowed_create_bempp.argtypes = [sz, vp, vp, vp, vp, vp, vp, vp, vp, vp, sz, sz, u8, vp,
+                                                              sz, sz, sz]
+    lib.create_lagrange_family_bempp.restype = vp
+    lib.create_lagrange_family_bempp.argtypes = [sz, u8, u8]
+    lib.create_raviart_thomas_family_bempp.restype = vp
+    lib.create_raviart_thomas_family_bempp.argtypes = [sz, u8, u8]
+    lib.function_space.restype = vp
+    lib.function_space.argtypes = [vp, vp]
+    lib.space_local_space.restype = vp
+    lib.space_local_space.argtypes = [vp]
+    for f in ("grid_t_free_bempp", "element_family_t_free_bempp", "space_t_free"):
+        getattr(lib, f).argtypes = [vp]
+    lib.space_get_local_dof_numbers_size.restype = sz
+    lib.space_get_local_dof_numbers_size.argtypes = [vp, sz, sz]
+    lib.space_get_local_dof_numbers.argtypes = [vp, sz, sz, vp]
+    lib.space_global_size.restype = sz
+    lib.space_global_size.argtypes = [vp]
+    lib.space_dtype.restype = u8
+    lib.space_dtype.argtypes = [vp]
+    lib.space_is_owned.restype = ctypes.c_int
+    lib.space_is_owned.argtypes = [vp, sz]
+    lib.space_cell_dofs.argtypes = [vp, sz, vp]
+    lib.grid_npoints.restype = sz
+    lib.grid_npoints.argtypes = [vp]
+    lib.grid_points_copy.argtypes = [vp, vp]
+    lib.grid_cells_copy.argtypes = [vp, vp]
+    lib.bempp_last_error.restype = ctypes.c_char_p
+    return lib
+
+
+def _entity_dofs(lib, sp, dim, index):
+    n = lib.space_get_local_dof_numbers_size(sp, dim, index)
+    out = np.zeros(n, dtype=np.uintp)
+    lib.space_get_local_dof_numbers(sp, dim, index, out.ctypes.data)
+    return [int(x) for x in out]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_ndgrid_grid_ingest(dtype):
+    """single_element_grid_borrowed_create_bempp / create_lagrange_family_bempp /
+    function_space (the entry points python/bempp/function_space.py binds): the grid equals the
+    one built from plain arrays, the spaces number DOFs identically, and entity DOFs follow the
+    topology's vertex numbering (here a random relabelling of the points)."""
+    lib = _raw_lib()
+    g0 = icosphere(2)
+    pts, cells = g0.points, g0.cells
+    perm = np.random.default_rng(5).permutation(len(pts))
+    args, keep = _ndgrid_args(pts, cells, perm, dtype)
+    gh = lib.single_element_grid_borrowed_create_bempp(*args)
+    assert gh, lib.bempp_last_error()
+    got = np.zeros_like(pts)
+    lib.grid_points_copy(gh, got.ctypes.data)
+    np.testing.assert_array_equal(got, np.asarray(pts, dtype=dtype).astype(np.float64))
+    gc = np.zeros_like(cells)
+    lib.grid_cells_copy(gh, gc.ctypes.data)
+    np.testing.assert_array_equal(gc, cells)
+    for degree, cont, nb in [(0, 1, 1), (1, 0, 3), (1, 1, 3)]:
+        fam = lib.create_lagrange_family_bempp(degree, cont, 3)
+        sp = lib.function_space(gh, fam)
+        assert sp, lib.bempp_last_error()
+        assert lib.space_dtype(sp) == 3
+        ref = function_space(g0, create_family(Family.Lagrange, degree, Continuity(cont)))
+        assert lib.space_global_size(sp) == ref.global_size
+        for c in (0, 17, len(cells) - 1):
+            d = np.zeros(nb, dtype=np.uintp)
+            lib.space_cell_dofs(sp, c, d.ctypes.data)
+            assert [int(x) for x in d] == ref.cell_dofs(c)
+        if degree == 1 and cont == 0:  # vertex DOFs, addressed by topology vertex
+            for p in (0, 5, len(pts) - 1):
+                c, m = np.argwhere(cells == p)[0]
+                assert _entity_dofs(lib, sp, 0, int(perm[p])) == [ref.cell_dofs(int(c))[m]]
+            assert _entity_dofs(lib, sp, 2, 3) == []
+        else:  # cell-interior DOFs
+            assert _entity_dofs(lib, sp, 2, 7) == ref.cell_dofs(7)
+            assert _entity_dofs(lib, sp, 0, 7) == []
+        assert lib.space_is_owned(sp, 0) == 1
+        loc = lib.space_local_space(sp)
+        assert lib.space_global_size(loc) == ref.global_size
+        lib.space_t_free(loc)
+        lib.space_t_free(sp)
+        lib.element_family_t_free_bempp(fam)
+    lib.grid_t_free_bempp(gh)
+
+
+def test_ndgrid_grid_ingest_errors():
+    lib = _raw_lib()
+    g0 = icosphere(1)
+    pts, cells = g0.points, g0.cells
+    ident = np.arange(len(pts))
+    for kwargs, msg in [({"tdim": 3}, "topological dimension"), ({"cell_type": 3}, "triangle"),
+                        ({}, None)]:
+        args, keep = _ndgrid_args(pts, cells, ident, **kwargs)
+        gh = lib.single_element_grid_borrowed_create_bempp(*args)
+        if msg is None:
+            assert gh
+            lib.grid_t_free_bempp(gh)
+        else:
+            assert not gh and msg in lib.bempp_last_error().decode()
+    # topology that is not a relabelling of the geometry (a cell's vertex order swapped)
+    args, keep = _ndgrid_args(pts, cells, ident)
+    tc = keep[2]
+    tc[0, [0, 1]] = tc[0, [1, 0]]
+    assert not lib.single_element_grid_borrowed_create_bempp(*args)
+    assert "relabelling" in lib.bempp_last_error().decode()
+    assert not lib.create_raviart_thomas_family_bempp(1, 0, 1)
+    assert "not supported" in lib.bempp_last_error().decode()
