@@ -362,7 +362,8 @@ const DeviceSpaceCache* space_blocks(FunctionSpace& s, int device) {
 // Canonical Sauter-Schwab rules for one Gauss order n, per device (kept for the
 // life of the process: a few hundred KB).
 struct RuleTable {
-  double* d = nullptr;
+  double* d = nullptr;    // 5 arrays of total points
+  double* aos = nullptr;  // one record of 6 per point (a0 a1 b0 b1 w pad)
   int total = 0;
   int off[4] = {0, 0, 0, 0};
   int n[4] = {0, 0, 0, 0};
@@ -387,6 +388,11 @@ const RuleTable& rule_table(int device, int n) {
     for (int i = 0; i < t.n[m]; ++i)
       for (int f = 0; f < 5; ++f) soa[(size_t)f * t.total + t.off[m] + i] = parts[m][5 * i + f];
   t.d = upload(soa);
+  std::vector<double> aos(6 * (size_t)t.total, 0.0);
+  for (int m = 1; m <= 3; ++m)
+    for (int i = 0; i < t.n[m]; ++i)
+      for (int f = 0; f < 5; ++f) aos[6 * (size_t)(t.off[m] + i) + f] = parts[m][5 * i + f];
+  t.aos = upload(aos);
   return tables.emplace(std::make_pair(device, n), t).first->second;
 }
 
@@ -670,6 +676,7 @@ Prepared prepare(const Assembler& a, const FunctionSpace& trial_c, const Functio
     S.pairs = grid_pairs(tgrid, tc, &P.npairs);
     S.npairs = P.npairs;
     S.rule = rt.d;
+    S.rule_aos = rt.aos;
     S.rule_total = rt.total;
     for (int m = 0; m < 4; ++m) {
       S.rule_off[m] = rt.off[m];

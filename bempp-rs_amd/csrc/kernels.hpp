@@ -74,6 +74,7 @@ struct SingularArgs {
   const int4* pairs;        // test cell, trial cell, shared count, permutation code
   int npairs;
   const double* rule;       // 5 arrays of rule_total: test xi0, test xi1, trial xi0, trial xi1, weight
+  const double* rule_aos;   // the same rule, one record of 6 per point: a0 a1 b0 b1 w pad
   int rule_total;
   int rule_off[4];          // indexed by shared count 1..3
   int rule_n[4];
@@ -717,65 +718,83 @@ __device__ __forceinline__ double wave_sum(double v) {
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int perm_digit(int code, int k) { return (code >> (2 * k)) & 3; }
 
+// A wave-uniform double moved to SGPRs (so that per-point FMAs can take it as their one
+// scalar operand instead of holding a VGPR copy).
+__device__ __forceinline__ double to_sgpr(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffLL));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
 template <int OP, bool HELM, bool P0P0, bool SMALL, typename Args>
 __device__ __forceinline__ void singular_pair(Args& A, const int pair, const int lane) {
   const auto pw = constant_view((const int*)A.pairs) + 4 * (size_t)pair;
   const int tc = pw[0], sc = pw[1], m = pw[2], code = pw[3];
   const auto tg = constant_view(A.test_geo) + (size_t)tc * kGeoStride;
   const auto sg = constant_view(A.trial_geo) + (size_t)sc * kGeoStride;
-  double tv[3][3], sv[3][3];
-#pragma unroll
-  for (int kk = 0; kk < 3; ++kk) {
-    const int pt = perm_digit(code, kk), ps = perm_digit(code >> 8, kk);
+  // x - y = D0 + E1 a0 + E2 a1 - F1 b0 - F2 b1: first-vertex difference (VGPRs) and edge
+  // vectors (SGPRs) of the canonically ordered test and trial cells
+  double D0[3], E1[3], E2[3], F1[3], F2[3];
+  {
+    const int t0 = perm_digit(code, 0), t1 = perm_digit(code, 1), t2 = perm_digit(code, 2);
+    const int s0 = perm_digit(code >> 8, 0), s1 = perm_digit(code >> 8, 1), s2 = perm_digit(code >> 8, 2);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      tv[kk][d] = tg[3 * pt + d];
-      sv[kk][d] = sg[3 * ps + d];
+      const double x0 = tg[3 * t0 + d], x1 = tg[3 * t1 + d], x2 = tg[3 * t2 + d];
+      const double y0 = sg[3 * s0 + d], y1 = sg[3 * s1 + d], y2 = sg[3 * s2 + d];
+      D0[d] = x0 - y0;
+      E1[d] = to_sgpr(x1 - x0);
+      E2[d] = to_sgpr(x2 - x0);
+      F1[d] = to_sgpr(y1 - y0);
+      F2[d] = to_sgpr(y2 - y0);
     }
   }
   const double nx[3] = {tg[9], tg[10], tg[11]};
   const double ny[3] = {sg[9], sg[10], sg[11]};
-  const double jj = tg[12] * sg[12];
   // selects rather than a dynamic index, so a struct copy of A stays in registers
   const int off = m == 1 ? A.rule_off[1] : (m == 2 ? A.rule_off[2] : A.rule_off[3]);
   const int n = m == 1 ? A.rule_n[1] : (m == 2 ? A.rule_n[2] : A.rule_n[3]);
-  const int tot = A.rule_total;
   constexpr int NB = P0P0 ? 1 : 3;
   cplx M[NB][NB];
 #pragma unroll
   for (int a = 0; a < NB; ++a)
 #pragma unroll
     for (int b = 0; b < NB; ++b) M[a][b] = {0.0, 0.0};
-  for (int pt = lane; pt < n; pt += 64) {
-    const double a0 = A.rule[off + pt], a1 = A.rule[tot + off + pt];
-    const double b0 = A.rule[2 * tot + off + pt], b1 = A.rule[3 * tot + off + pt];
-    const double w = A.rule[4 * tot + off + pt] * jj;
-    const double lt[3] = {1.0 - a0 - a1, a0, a1};
-    const double ls[3] = {1.0 - b0 - b1, b0, b1};
+  // one pointer per lane into the interleaved rule: the five loads share it (immediate offsets)
+  const double* rp = A.rule_aos + (size_t)(off + lane) * 6;
+  for (int pt = lane; pt < n; pt += 64, rp += 6 * 64) {
+    const double2 ta = *(const double2*)rp, tb = *(const double2*)(rp + 2);
+    const double a0 = ta.x, a1 = ta.y, b0 = tb.x, b1 = tb.y;
+    const double w = rp[4];  // the Jacobians are applied once per pair below
     double d[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
-      d[c] = (tv[0][c] * lt[0] + tv[1][c] * lt[1] + tv[2][c] * lt[2]) -
-             (sv[0][c] * ls[0] + sv[1][c] * ls[1] + sv[2][c] * ls[2]);
-    cplx kv = kernel_scalar<OP, HELM, SMALL>(d[0], d[1], d[2], A.k, nx, ny, 0.0);
-    kv.re *= w;
-    kv.im *= w;
-    if constexpr (P0P0) {
-      M[0][0].re += kv.re;
-      M[0][0].im += kv.im;
+    for (int c = 0; c < 3; ++c) d[c] = fma(E1[c], a0, fma(E2[c], a1, fma(-F1[c], b0, fma(-F2[c], b1, D0[c]))));
+    if constexpr (P0P0 && (OP == 0 || OP == 3)) {
+      green_accumulate<HELM, SMALL>(d[0], d[1], d[2], A.k, 0.0, w, M[0][0]);
     } else {
+      const cplx kv = kernel_scalar<OP, HELM, SMALL>(d[0], d[1], d[2], A.k, nx, ny, 0.0);
+      if constexpr (P0P0) {
+        M[0][0].re = fma(kv.re, w, M[0][0].re);
+        M[0][0].im = fma(kv.im, w, M[0][0].im);
+      } else {
+        const double lt[3] = {1.0 - a0 - a1, a0, a1};
+        const double ls[3] = {1.0 - b0 - b1, b0, b1};
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const double fa = A.nbt == 3 ? lt[a] : (a == 0 ? 1.0 : 0.0);
+        for (int a = 0; a < 3; ++a) {
+          const double fa = (A.nbt == 3 ? lt[a] : (a == 0 ? 1.0 : 0.0)) * w;
+          const cplx u = {kv.re * fa, kv.im * fa};
 #pragma unroll
-        for (int b = 0; b < 3; ++b) {
-          const double f = fa * (A.nbs == 3 ? ls[b] : (b == 0 ? 1.0 : 0.0));
-          M[a][b].re = fma(kv.re, f, M[a][b].re);
-          M[a][b].im = fma(kv.im, f, M[a][b].im);
+          for (int b = 0; b < 3; ++b) {
+            const double fb = A.nbs == 3 ? ls[b] : (b == 0 ? 1.0 : 0.0);
+            M[a][b].re = fma(u.re, fb, M[a][b].re);
+            M[a][b].im = fma(u.im, fb, M[a][b].im);
+          }
         }
       }
     }
   }
+  const double jj = tg[12] * sg[12];
   cplx Ml[3][3];
   cplx msum = {0.0, 0.0};
 #pragma unroll
@@ -786,8 +805,8 @@ __device__ __forceinline__ void singular_pair(Args& A, const int pair, const int
   for (int a = 0; a < NB; ++a)
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      Ml[a][b].re = wave_sum(M[a][b].re);
-      Ml[a][b].im = HELM ? wave_sum(M[a][b].im) : 0.0;
+      Ml[a][b].re = wave_sum(M[a][b].re) * jj;
+      Ml[a][b].im = HELM ? wave_sum(M[a][b].im) * jj : 0.0;
       msum.re += Ml[a][b].re;
       msum.im += Ml[a][b].im;
     }
