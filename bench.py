@@ -16,6 +16,8 @@ ROOT = Path(__file__).resolve().parent
 sys.path[:0] = [str(ROOT / "bempp-rs_amd"), str(ROOT / "tests")]
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (and matrix) peak
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector peak
+HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E
 # nominal flops per regular quadrature-point pair of Helmholtz SL (x-y: 3, r^2: 5, sqrt: 1,
 # 1/r: 1, kr: 1, sin, cos: 2, e^{ikr}/r: 2, weight: 1, complex accumulate: 4) and per pair epilogue
 FLOP_PER_QPP = 20
@@ -347,7 +349,8 @@ def main():
         key = "complex64" if mdt == np.complex64 else "complex128"
         mixed[key] = {"value": entries / (mms * 1e-3), "ms_per_step": mms, "speedup_vs_fp64": ms_per_step / mms,
                       "sample_rel_err_vs_oracle": merr, "regular_kernels_ms": float(np.mean(mreg)),
-                      "adjacent_kernel_ms": float(np.mean(msing))}
+                      "adjacent_kernel_ms": float(np.mean(msing)),
+                      "written_bytes_per_entry": 8 if mdt == np.complex64 else 16}
         print(f"[bench] mixed {key}: {mms:.3f} ms/step, far+near kernels {np.mean(mreg):.3f} ms, adjacent "
               f"{np.mean(msing):.3f} ms, sample rel err {merr:.2e}", file=sys.stderr, flush=True)
         if mout is not out:
@@ -358,6 +361,15 @@ def main():
     n_reg = entries - n_adj
     flops_reg = n_reg * (36 * FLOP_PER_QPP + FLOP_PER_PAIR_EPILOGUE)
     achieved = flops_reg / (np.mean(reg_ms) * 1e-3) / 1e12
+    # the tile kernel writes every entry once (complex128): algorithmic write rate
+    write_gbps = entries * 16 / (np.mean(reg_ms) * 1e-3) / 1e9
+    for m in mixed.values():  # the same nominal flops and entries over the far+near kernels' time
+        m["roofline_fp32"] = {
+            "achieved": flops_reg / (m["regular_kernels_ms"] * 1e-3) / 1e12, "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": flops_reg / (m["regular_kernels_ms"] * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
+            "hbm_write_GBps": entries * m["written_bytes_per_entry"] / (m["regular_kernels_ms"] * 1e-3) / 1e9,
+            "hbm_write_frac": entries * m["written_bytes_per_entry"] / (m["regular_kernels_ms"] * 1e-3) / 1e9
+            / HBM_PEAK_GBPS}
 
     batched = None
     if not args.no_batch:
@@ -398,6 +410,7 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": TILE_KERNEL_TRAFFIC_L5_K5 if (args.level, args.k) == (5, 5.0) else None,
+                     "hbm_write_GBps": write_gbps, "hbm_write_frac": write_gbps / HBM_PEAK_GBPS,
                      "basis": f"{n_reg} regular pairs x (36 x {FLOP_PER_QPP} + {FLOP_PER_PAIR_EPILOGUE}) nominal "
                               "flop per launch / mean regular-kernel HIP-event time"},
         "cpu_baseline": cpu,
