@@ -77,10 +77,18 @@ __device__ __forceinline__ void sincos_small_f32(float x, float* s, float* c) {
 // sin/cos in FP32 for moderate |x| (the far-pair phase k delta, |k delta| <= |k| (rad_i + rad_j)):
 // x = n pi/2 + t with pi/2 split in two FP32 parts (each n C fused exactly), |t| <= pi/4,
 // Taylor series to t^9 / t^10 (truncation below 2e-9), quadrant by n mod 4.
+__device__ __forceinline__ void sincos_f32_quadrant(float t, int q, float* s, float* c);
+
 __device__ __forceinline__ void sincos_f32_wide(float x, float* s, float* c) {
   const float n = __builtin_rintf(x * 0.63661977f);
   float t = fmaf(-n, 1.5707964f, x);
   t = fmaf(-n, -4.3711388e-8f, t);
+  sincos_f32_quadrant(t, (int)n, s, c);
+}
+
+// sin/cos of n pi/2 + t for |t| <= pi/4 given t and the quadrant count n: Taylor series to
+// t^9 / t^10 (truncation below 2e-9), quadrant by n mod 4.
+__device__ __forceinline__ void sincos_f32_quadrant(float t, int q, float* s, float* c) {
   const float t2 = t * t;
   float ps = fmaf(t2, 2.7557319e-6f, -1.9841270e-4f);
   ps = fmaf(ps, t2, 8.3333333e-3f);
@@ -93,7 +101,6 @@ __device__ __forceinline__ void sincos_f32_wide(float x, float* s, float* c) {
   const float ct = fmaf(pc, t2, 1.0f);
   // quadrant q = n mod 4: sin x = (s, c, -s, -c)[q], cos x = (c, -s, -c, s)[q]; the
   // negations flip the sign bit (bit 1 of q for sin, of q + 1 for cos)
-  const int q = (int)n;
   const bool swap = q & 1;
   const float sv = swap ? ct : st, cv = swap ? st : ct;
   *s = __int_as_float(__float_as_int(sv) ^ ((q & 2) << 30));
@@ -202,9 +209,9 @@ __device__ __forceinline__ void p0_tile_mixed(Args& A, const int2 tile, int symm
     mp[p][2] = -2.0f * sig * o.z;
     oo[p] = sig * fmaf(o.x, o.x, fmaf(o.y, o.y, o.z * o.z));
   }
-  double wt[NC];  // test weight of each class (rule weight x Jacobian)
+  float wt[NC];  // test weight of each class (rule weight x Jacobian)
 #pragma unroll
-  for (int c = 0; c < NC; ++c) wt[c] = A.test_pts[((size_t)il * Q + class_point(Q, c)) * 4 + 3];
+  for (int c = 0; c < NC; ++c) wt[c] = ((const float4*)A.test_pts32)[(size_t)il * Q + class_point(Q, c)].w;
   const double* tgeo = A.test_geo + (size_t)il * kGeoStride;
   const float nx[3] = {(float)tgeo[9], (float)tgeo[10], (float)tgeo[11]};
   const double ct[3] = {tgeo[kGeoCentroid], tgeo[kGeoCentroid + 1], tgeo[kGeoCentroid + 2]};
@@ -291,30 +298,37 @@ __device__ __forceinline__ void p0_tile_mixed(Args& A, const int2 tile, int symm
               far_accumulate_f32<OP, false, PH>(s, kDN ? dnp - nq[q] : 0.0f, R2f, Rf, 0.0f, 1.0f, cl);
           }
         }
+        // class weights, length scaling and e^{ikR} in FP32 (relative error ~1e-7 per entry);
+        // only the reduction kR = n pi/2 + t needs FP64
+        cplxf af = {0.0f, 0.0f};
 #pragma unroll
         for (int a = 0; a < NC; ++a) {
-          cplx row = {0.0, 0.0};
+          cplxf row = {0.0f, 0.0f};
 #pragma unroll
           for (int b = 0; b < NC; ++b) {
-            const double ws = A.trial_pts[((size_t)j * Q + class_point(Q, b)) * 4 + 3];
-            row.re = fma((double)cls[a][b].re, ws, row.re);
-            row.im = fma((double)cls[a][b].im, ws, row.im);
+            const float ws = sp[4 * class_point(Q, b) + 3];
+            row.re = fmaf(cls[a][b].re, ws, row.re);
+            row.im = fmaf(cls[a][b].im, ws, row.im);
           }
-          acc.re = fma(row.re, wt[a], acc.re);
-          acc.im = fma(row.im, wt[a], acc.im);
+          af.re = fmaf(row.re, wt[a], af.re);
+          af.im = fmaf(row.im, wt[a], af.im);
         }
         if constexpr (HELM) {
           // undo the length scaling (1/r = kappa / r'), conjugate for k < 0, rotate by e^{ikR}
-          const double ks = kDN ? kappa * kappa * kappa : kappa;
-          acc = {acc.re * ks, A.k < 0.0 ? -acc.im * ks : acc.im * ks};
-          cplx e;
-          sincos_cw(A.k * R, &e.im, &e.re);
-          acc = cmul(acc, e);
+          const double kR = A.k * R;
+          const double n = rint(kR * 0.63661977236758134);
+          const double t = fma(-n, 6.123233995736766e-17, fma(-n, 1.5707963267948966, kR));
+          float sn, cs;
+          sincos_f32_quadrant((float)t, (int)n, &sn, &cs);
+          const float ks = (float)(kDN ? kappa * kappa * kappa : kappa);
+          const float im = A.k < 0.0 ? -af.im : af.im;
+          af = {fmaf(af.re, cs, -im * sn) * ks, fmaf(af.re, sn, im * cs) * ks};
         }
         if constexpr (OP == 3) {
-          const double nn = HELM ? -A.k * A.k * (tgeo[9] * sg[9] + tgeo[10] * sg[10] + tgeo[11] * sg[11]) : 0.0;
-          acc = {nn * acc.re, nn * acc.im};
+          const float nn = HELM ? (float)(-A.k * A.k * (tgeo[9] * sg[9] + tgeo[10] * sg[10] + tgeo[11] * sg[11])) : 0.0f;
+          af = {nn * af.re, nn * af.im};
         }
+        acc = {(double)af.re, (double)af.im};
       }
       const cplx v = {acc.re * kInv4Pi, acc.im * kInv4Pi};
       if (i_ok) store_value(A.out, (long long)i + A.ld * (long long)j, v, A.out_type, A.mode);
