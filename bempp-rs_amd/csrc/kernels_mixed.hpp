@@ -538,22 +538,29 @@ __device__ __forceinline__ void p1_tile_mixed(const RegularArgs& A, const int2 t
           }
       }
     }
-    // FP64: Jacobians, the length scaling (1/r = kappa / r'), k < 0 (conjugate), e^{ikR}
+    // Jacobians, the length scaling (1/r = kappa / r'), k < 0 (conjugate) and e^{ikR} in FP32;
+    // only the reduction kR = n pi/2 + t needs FP64
     double scale = tgeo[12] * sg[12];
     if constexpr (HELM) scale *= kDN ? kappa * kappa * kappa : kappa;
-    cplx e = {scale, 0.0};
+    float er = (float)scale, ei = 0.0f;
     if constexpr (HELM) {
-      sincos_cw(A.k * R, &e.im, &e.re);
-      e = {e.re * scale, e.im * scale};
+      const double kR = A.k * R;
+      const double n = rint(kR * 0.63661977236758134);
+      const double t = fma(-n, 6.123233995736766e-17, fma(-n, 1.5707963267948966, kR));
+      float sn, cs;
+      sincos_f32_quadrant((float)t, (int)n, &sn, &cs);
+      ei = er * sn;
+      er = er * cs;
     }
-    const double sk = HELM && A.k < 0.0 ? -1.0 : 1.0;
+    const float sk = HELM && A.k < 0.0 ? -1.0f : 1.0f;
     cplx M64[3][3];
     cplx msum = {0.0, 0.0};
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
       for (int b = 0; b < 3; ++b) {
-        M64[a][b] = cmul({(double)M[a][b].re, sk * (double)M[a][b].im}, e);
+        const float mr = M[a][b].re, mi = sk * M[a][b].im;
+        M64[a][b] = {(double)fmaf(mr, er, -mi * ei), (double)fmaf(mr, ei, mi * er)};
         msum.re += M64[a][b].re;
         msum.im += M64[a][b].im;
       }
