@@ -35,8 +35,10 @@ struct DeviceGridCache {
   std::vector<AdjacentPair> host_pairs;
   double max_radius = 0.0;        // largest centroid-to-vertex distance of any cell
   std::map<int, float*> pts32;    // regular point count -> [nc][Q][4] FP32 offsets from the centroid, w*jdet
-  int2* near = nullptr;           // non-adjacent pairs closer than the mixed-precision far bound (superset)
-  int nnear = -1;
+  int2* near = nullptr;           // non-adjacent pairs closer than the mixed-precision far bound (superset),
+  int nnear = -1;                 // sorted by (test block, trial block) of 64 cells
+  int4* near_segs = nullptr;      // per block pair with near pairs: test block, trial block, first, count
+  int nsegs = 0;
 };
 
 struct DeviceSpaceCache {
@@ -58,6 +60,7 @@ void free_device_grid_cache(DeviceGridCache* c) {
     for (auto& kv : c->pts) (void)hipFree(kv.second);
     for (auto& kv : c->pts32) (void)hipFree(kv.second);
     (void)hipFree(c->near);
+    (void)hipFree(c->near_segs);
     (void)hipFree(c->pairs);
     (void)hipSetDevice(cur);
   }
@@ -284,14 +287,27 @@ std::vector<int2> near_pairs_host(const Grid& g) {
   return out;
 }
 
-const int2* grid_near_pairs(Grid& g, DeviceGridCache* c, int* n) {
+const int2* grid_near_pairs(Grid& g, DeviceGridCache* c, int* n, const int4** segs, int* nsegs) {
   std::lock_guard<std::mutex> lk(g.cache_mutex);
   if (c->nnear < 0) {
     std::vector<int2> v = near_pairs_host(g);
+    // group by (test block, trial block) of kTile cells: the P1 near kernel sums each group in LDS
+    auto key = [](const int2& p) { return std::make_tuple(p.x / dev::kTile, p.y / dev::kTile, p.x, p.y); };
+    std::sort(v.begin(), v.end(), [&](const int2& a, const int2& b) { return key(a) < key(b); });
+    std::vector<int4> sg;
+    for (size_t i = 0; i < v.size(); ++i) {
+      const int tx = v[i].x / dev::kTile, ty = v[i].y / dev::kTile;
+      if (sg.empty() || sg.back().x != tx || sg.back().y != ty) sg.push_back(make_int4(tx, ty, (int)i, 0));
+      ++sg.back().w;
+    }
     c->near = upload(v);
+    c->near_segs = upload(sg);
     c->nnear = (int)v.size();
+    c->nsegs = (int)sg.size();
   }
   *n = c->nnear;
+  if (segs) *segs = c->near_segs;
+  if (nsegs) *nsegs = c->nsegs;
   return c->near;
 }
 
@@ -631,6 +647,8 @@ Prepared prepare(const Assembler& a, const FunctionSpace& trial_c, const Functio
   P.mixed = a.options.precision == kMixed && P.same_grid;
   P.near = nullptr;
   P.nnear = 0;
+  P.near_segs = nullptr;
+  P.nsegs = 0;
   R.test_pts32 = R.trial_pts32 = nullptr;
   if (P.mixed) {
     // the mixed P0 tiles sum point pairs per weight class (kernels_mixed.hpp)
@@ -638,7 +656,7 @@ Prepared prepare(const Assembler& a, const FunctionSpace& trial_c, const Functio
       if (w[p] != w[dev::class_point(P.Q, dev::weight_class(P.Q, p))])
         throw Error("mixed precision: quadrature rule weights do not follow the orbit classes");
     R.test_pts32 = R.trial_pts32 = grid_points32_for_rule(tgrid, tc, P.Q, xi, w);
-    P.near = grid_near_pairs(tgrid, tc, &P.nnear);
+    P.near = grid_near_pairs(tgrid, tc, &P.nnear, &P.near_segs, &P.nsegs);
   }
   R.test_geo = tc->geo;
   R.trial_geo = sc->geo;
@@ -787,7 +805,7 @@ void assemble_device(const Assembler& a, const FunctionSpace& trial, const Funct
         const int2* tiles = tile_list(P.device, nta, ntb, sym, &ntiles);
         if (P.mixed)
           launch_mixed_lds(a.op, a.kind == kHelmholtz, P.tier, P.Q, dim3(ntiles), P.stream, R, tiles, sym ? 1 : 0,
-                           lds, P.near, P.nnear);
+                           lds, P.near, P.near_segs, P.nsegs);
         else
           launch_regular(a.op, a.kind == kHelmholtz, P.tier, 2, P.Q, dim3(ntiles), P.stream, R, tiles,
                          sym ? 1 : 0, lds);

@@ -1018,6 +1018,8 @@ struct Prepared {
   bool mixed;           // FP32 far pairs + FP64 near-pair list (P0 x P0 on one grid)
   const int2* near;
   int nnear;
+  const int4* near_segs;  // near pairs grouped by (test block, trial block): tx, ty, first, count
+  int nsegs;
   hipStream_t stream;
   int Q;
   bool same_grid;

@@ -367,7 +367,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) r
 // FP32; the Jacobians, the length scaling and e^{ikR} are applied once per cell pair in
 // FP64. The Laplace hypersingular operator needs only sum_pq w_p w_q K_pq (its P1 surface
 // curls are constant on a cell). Near and adjacent pairs contribute nothing here: near
-// pairs come from near_pair_p1_kernel in FP64, adjacent pairs from the Duffy kernel.
+// pairs come from near_pair_p1_tiled in FP64, adjacent pairs from the Duffy kernel.
 // ---------------------------------------------------------------------------
 
 // FP32 kernel value of a far point pair without weights, e^{ikR} and 1/(4 pi). Helmholtz:
@@ -587,44 +587,54 @@ __global__ void __launch_bounds__(256) regular_tiled_lds_mixed(const RegularArgs
   p1_tile_mixed<Q, OP, HELM, PH>(A, tiles[blockIdx.x], symmetric, acc);
 }
 
-// FP64 evaluation of the near (non-adjacent) cell pairs with a P1 side, one thread per
-// pair; the local block is added with atomics at the DOF indices.
+// FP64 evaluation of the near (non-adjacent) cell pairs with a P1 side, grouped by
+// (test block, trial block) of kTile cells: one workgroup per group sums its pairs' local
+// blocks in LDS at block-local DOF indices (as regular_tiled_lds does) and adds the touched
+// entries to the output with one global atomic each.
 template <int Q, int OP, bool HELM, int CENTRED>
-__global__ void __launch_bounds__(256) near_pair_p1_kernel(const RegularArgs A, const int2* __restrict__ pairs,
-                                                           int npairs) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= npairs) return;
-  const int2 pr = pairs[t];
-  const int i = pr.x, j = pr.y;
-  const double* tg = A.test_geo + (size_t)i * kGeoStride;
-  const double* sg = A.trial_geo + (size_t)j * kGeoStride;
-  double R2;
-  if (far_pair(tg + kGeoCentroid, sg + kGeoCentroid, tg[kGeoRadius], sg[kGeoRadius], &R2)) return;
-  double xp[Q][3], wp[Q], sp[Q * 4];
+__global__ void __launch_bounds__(256) near_pair_p1_tiled(const RegularArgs A, const int2* __restrict__ pairs,
+                                                          const int4* __restrict__ segs) {
+  extern __shared__ double2 acc[];
+  const int4 sgm = segs[blockIdx.x];
+  const int t0 = A.test_blk_off[sgm.x], ndt = A.test_blk_off[sgm.x + 1] - t0;
+  const int s0 = A.trial_blk_off[sgm.y], nds = A.trial_blk_off[sgm.y + 1] - s0;
+  for (int e = threadIdx.x; e < ndt * nds; e += blockDim.x) acc[e] = make_double2(0.0, 0.0);
+  __syncthreads();
+  for (int e = threadIdx.x; e < sgm.w; e += blockDim.x) {
+    const int2 pr = pairs[sgm.z + e];
+    const int i = pr.x, j = pr.y;
+    const double* tg = A.test_geo + (size_t)i * kGeoStride;
+    const double* sg = A.trial_geo + (size_t)j * kGeoStride;
+    double R2;
+    if (far_pair(tg + kGeoCentroid, sg + kGeoCentroid, tg[kGeoRadius], sg[kGeoRadius], &R2)) continue;
+    double xp[Q][3], wp[Q], sp[Q * 4];
 #pragma unroll
-  for (int p = 0; p < Q; ++p) {
-    const double* s = A.test_pts + ((size_t)i * Q + p) * 4;
-    xp[p][0] = s[0];
-    xp[p][1] = s[1];
-    xp[p][2] = s[2];
-    wp[p] = s[3];
-  }
-#pragma unroll
-  for (int m = 0; m < Q * 4; ++m) sp[m] = A.trial_pts[(size_t)j * Q * 4 + m];
-  const double nx[3] = {tg[9], tg[10], tg[11]};
-  const double ct[3] = {tg[kGeoCentroid], tg[kGeoCentroid + 1], tg[kGeoCentroid + 2]};
-  cplx M[3][3], msum;
-  p1_local<Q, OP, HELM, CENTRED>(A, xp, wp, nx, ct, sp, sg, M, msum);
-#pragma unroll
-  for (int b = 0; b < 3; ++b)
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      if (a >= A.nbt || b >= A.nbs) continue;
-      const cplx v = local_entry<OP, HELM>(M, a, b, msum, A.nbt, A.nbs, tg, sg, a, b, A.k);
-      const long long row = A.test_dofs[(size_t)i * A.nbt + a];
-      const long long col = A.trial_dofs[(size_t)j * A.nbs + b];
-      store_value(A.out, row + A.ld * col, v, A.out_type, kAtomic);
+    for (int p = 0; p < Q; ++p) {
+      const double* s = A.test_pts + ((size_t)i * Q + p) * 4;
+      xp[p][0] = s[0];
+      xp[p][1] = s[1];
+      xp[p][2] = s[2];
+      wp[p] = s[3];
     }
+#pragma unroll
+    for (int m = 0; m < Q * 4; ++m) sp[m] = A.trial_pts[(size_t)j * Q * 4 + m];
+    const double nx[3] = {tg[9], tg[10], tg[11]};
+    const double ct[3] = {tg[kGeoCentroid], tg[kGeoCentroid + 1], tg[kGeoCentroid + 2]};
+    cplx M[3][3], msum;
+    p1_local<Q, OP, HELM, CENTRED>(A, xp, wp, nx, ct, sp, sg, M, msum);
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        if (a >= A.nbt || b >= A.nbs) continue;
+        const cplx v = local_entry<OP, HELM>(M, a, b, msum, A.nbt, A.nbs, tg, sg, a, b, A.k);
+        double2* slot = acc + (A.trial_ldof[(size_t)j * A.nbs + b] * ndt + A.test_ldof[(size_t)i * A.nbt + a]);
+        atomicAdd(&slot->x, v.re);
+        if constexpr (HELM) atomicAdd(&slot->y, v.im);
+      }
+  }
+  __syncthreads();
+  flush_tile(A, acc, t0, ndt, s0, nds, false);
 }
 
 // FP64 evaluation of a list of near (non-adjacent) P0 x P0 cell pairs, one
@@ -721,6 +731,6 @@ __global__ void __launch_bounds__(256) batched_near_kernel(const RegularArgs* __
 
 // p1mixed.hip
 void launch_mixed_lds(int op, bool helm, int tier, int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& A,
-                      const int2* tiles, int sym, size_t lds, const int2* near, int nnear);
+                      const int2* tiles, int sym, size_t lds, const int2* near, const int4* near_segs, int nsegs);
 
 }  // namespace bempp_amd

@@ -14,12 +14,11 @@ namespace {
 
 template <int OP, bool HELM, int TIER>
 void launch_q(int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& A, const int2* tiles, int sym, size_t lds,
-              const int2* near, int nnear) {
-  const dim3 ng((unsigned)((nnear + 255) / 256));
+              const int2* near, const int4* segs, int nsegs) {
   constexpr int NC = HELM && TIER > 0 ? 1 : 0;  // the FP64 near pairs use the centred tier 1 when possible
 #define BEMPP_LAUNCH(QQ)                                                                          \
   dev::regular_tiled_lds_mixed<QQ, OP, HELM, TIER><<<grid, 256, lds, s>>>(A, tiles, sym);         \
-  if (nnear > 0) dev::near_pair_p1_kernel<QQ, OP, HELM, NC><<<ng, 256, 0, s>>>(A, near, nnear);
+  if (nsegs > 0) dev::near_pair_p1_tiled<QQ, OP, HELM, NC><<<nsegs, 256, lds, s>>>(A, near, segs);
   switch (Q) {
     case 1: BEMPP_LAUNCH(1); break;
     case 3: BEMPP_LAUNCH(3); break;
@@ -32,12 +31,12 @@ void launch_q(int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& A, const 
 
 template <bool HELM, int TIER>
 void launch_op(int op, int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& A, const int2* tiles, int sym,
-               size_t lds, const int2* near, int nnear) {
+               size_t lds, const int2* near, const int4* segs, int nsegs) {
   switch (op) {
-    case kSL: launch_q<kSL, HELM, TIER>(Q, grid, s, A, tiles, sym, lds, near, nnear); break;
-    case kDL: launch_q<kDL, HELM, TIER>(Q, grid, s, A, tiles, sym, lds, near, nnear); break;
-    case kADL: launch_q<kADL, HELM, TIER>(Q, grid, s, A, tiles, sym, lds, near, nnear); break;
-    case kHS: launch_q<kHS, HELM, TIER>(Q, grid, s, A, tiles, sym, lds, near, nnear); break;
+    case kSL: launch_q<kSL, HELM, TIER>(Q, grid, s, A, tiles, sym, lds, near, segs, nsegs); break;
+    case kDL: launch_q<kDL, HELM, TIER>(Q, grid, s, A, tiles, sym, lds, near, segs, nsegs); break;
+    case kADL: launch_q<kADL, HELM, TIER>(Q, grid, s, A, tiles, sym, lds, near, segs, nsegs); break;
+    case kHS: launch_q<kHS, HELM, TIER>(Q, grid, s, A, tiles, sym, lds, near, segs, nsegs); break;
     default: throw Error("operator not supported by the dense assembler");
   }
 }
@@ -45,13 +44,13 @@ void launch_op(int op, int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& 
 }  // namespace
 
 // Mixed precision with a P1 side: FP32 far-pair tiles with LDS accumulation, then the
-// FP64 near-pair list. tier: phase tier of the assembly (0 range-reduced, 1, 2 centred).
+// FP64 near pairs, summed in LDS per (test block, trial block) group. tier: phase tier of the assembly (0 range-reduced, 1, 2 centred).
 void launch_mixed_lds(int op, bool helm, int tier, int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& A,
-                      const int2* tiles, int sym, size_t lds, const int2* near, int nnear) {
-  if (!helm) launch_op<false, 1>(op, Q, grid, s, A, tiles, sym, lds, near, nnear);
-  else if (tier == 2) launch_op<true, 2>(op, Q, grid, s, A, tiles, sym, lds, near, nnear);
-  else if (tier == 1) launch_op<true, 1>(op, Q, grid, s, A, tiles, sym, lds, near, nnear);
-  else launch_op<true, 0>(op, Q, grid, s, A, tiles, sym, lds, near, nnear);
+                      const int2* tiles, int sym, size_t lds, const int2* near, const int4* segs, int nsegs) {
+  if (!helm) launch_op<false, 1>(op, Q, grid, s, A, tiles, sym, lds, near, segs, nsegs);
+  else if (tier == 2) launch_op<true, 2>(op, Q, grid, s, A, tiles, sym, lds, near, segs, nsegs);
+  else if (tier == 1) launch_op<true, 1>(op, Q, grid, s, A, tiles, sym, lds, near, segs, nsegs);
+  else launch_op<true, 0>(op, Q, grid, s, A, tiles, sym, lds, near, segs, nsegs);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) throw Error(std::string("mixed-precision P1 launch failed: ") + hipGetErrorString(e));
 }
