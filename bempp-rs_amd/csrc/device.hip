@@ -742,12 +742,12 @@ void assemble_device(const Assembler& a, const FunctionSpace& trial, const Funct
   const bool p0p0 = test.nb == 1 && trial.nb == 1;
   const long long n_out = (long long)test.size * (long long)trial.size * (is_complex(a.dtype) ? 2 : 1);
   if (!p0p0 && !accumulate) {
-    if (a.dtype == kF32 || a.dtype == kC32) {
-      HIP_CHECK(hipMemsetAsync(dev_out, 0, (size_t)n_out * 4, P.stream));
-    } else {
+    // a fill kernel, not a memset node: the caller may capture this stream in a graph
+    if (a.dtype == kF32 || a.dtype == kC32)
+      dev::fill_zero_kernel<<<2048, 256, 0, P.stream>>>((float*)dev_out, n_out);
+    else
       dev::fill_zero_kernel<<<2048, 256, 0, P.stream>>>((double*)dev_out, n_out);
-      HIP_CHECK(hipGetLastError());
-    }
+    HIP_CHECK(hipGetLastError());
   }
   // regular pairs
   dev::RegularArgs R = P.R;
@@ -1027,7 +1027,8 @@ void assemble_potential_device(const Assembler& a, const FunctionSpace& space_c,
   P.mode = space.nb == 1 ? dev::kAccumulate : dev::kAtomic;
   if (!accumulate) {
     const long long n_out = (long long)npoints * (long long)space.size * (is_complex(a.dtype) ? 2 : 1);
-    if (a.dtype == kF32 || a.dtype == kC32) HIP_CHECK(hipMemsetAsync(dev_out, 0, (size_t)n_out * 4, s));
+    // a fill kernel, not a memset node: the caller may capture this stream in a graph
+    if (a.dtype == kF32 || a.dtype == kC32) dev::fill_zero_kernel<<<2048, 256, 0, s>>>((float*)dev_out, n_out);
     else dev::fill_zero_kernel<<<2048, 256, 0, s>>>((double*)dev_out, n_out);
     HIP_CHECK(hipGetLastError());
   }
