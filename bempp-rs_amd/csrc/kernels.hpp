@@ -159,6 +159,22 @@ __device__ __forceinline__ void green_accumulate(double dx, double dy, double dz
   }
 }
 
+// Overwriting stores of the two complex output types take a short path; every other
+// combination of output type and mode goes through store_value. kind: 1 complex64 store,
+// 2 complex128 store, 0 other (wave-uniform).
+__device__ __forceinline__ int fast_store_kind(int out_type, int mode) {
+  return mode != kStore ? 0 : out_type == kOutC64 ? 1 : out_type == kOutC128 ? 2 : 0;
+}
+__device__ __forceinline__ void store_value(void* out, long long idx, cplx v, int out_type, int mode);
+__device__ __forceinline__ void store_entry(void* out, long long idx, cplx v, int kind, int out_type, int mode) {
+  if (kind == 1)
+    ((float2*)out)[idx] = make_float2((float)v.re, (float)v.im);
+  else if (kind == 2)
+    ((double2*)out)[idx] = make_double2(v.re, v.im);
+  else
+    store_value(out, idx, v, out_type, mode);
+}
+
 __device__ __forceinline__ void store_value(void* out, long long idx, cplx v, int out_type, int mode) {
   if (mode == kAtomic) {
     switch (out_type) {
@@ -599,6 +615,7 @@ __device__ __forceinline__ void p0_tile(Args& A, const int2 tile, int symmetric,
   const double ct[3] = {tgeo[kGeoCentroid], tgeo[kGeoCentroid + 1], tgeo[kGeoCentroid + 2]};
   const int tv0 = A.test_vid[4 * il], tv1 = A.test_vid[4 * il + 1], tv2 = A.test_vid[4 * il + 2];
   const bool mirror = symmetric && tile.x != tile.y;
+  const int skind = fast_store_kind(A.out_type, A.mode);
   const auto trial_pts = constant_view(A.trial_pts);
   const auto trial_geo = constant_view(A.trial_geo);
   const auto trial_vid = constant_view(A.trial_vid);
@@ -639,7 +656,7 @@ __device__ __forceinline__ void p0_tile(Args& A, const int2 tile, int symmetric,
       }
       // adjacent pairs get 0 here (their Duffy value is added by the singular kernel)
       const cplx v = {acc.re * kInv4Pi, acc.im * kInv4Pi};
-      if (i_ok) store_value(A.out, (long long)i + A.ld * (long long)j, v, A.out_type, A.mode);
+      if (i_ok) store_entry(A.out, (long long)i + A.ld * (long long)j, v, skind, A.out_type, A.mode);
       if (mirror) lds[wave][jj][lane] = make_double2(v.re, v.im);
     }
     if (mirror) {  // block-uniform
@@ -652,7 +669,7 @@ __device__ __forceinline__ void p0_tile(Args& A, const int2 tile, int symmetric,
         const int ic = tile.x * kTile + col;  // column of the mirrored entry (a test cell)
         const double2 val = lds[wave][row][col];
         if (jr < A.n_trial && ic < A.n_test)
-          store_value(A.out, (long long)jr + A.ld * (long long)ic, {val.x, val.y}, A.out_type, A.mode);
+          store_entry(A.out, (long long)jr + A.ld * (long long)ic, {val.x, val.y}, skind, A.out_type, A.mode);
       }
       __syncthreads();
     }

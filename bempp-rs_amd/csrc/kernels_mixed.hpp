@@ -218,6 +218,7 @@ __device__ __forceinline__ void p0_tile_mixed(Args& A, const int2 tile, int symm
   const double trad = tgeo[kGeoRadius];
   const int tv0 = A.test_vid[4 * il], tv1 = A.test_vid[4 * il + 1], tv2 = A.test_vid[4 * il + 2];
   const bool mirror = symmetric && tile.x != tile.y;
+  const int skind = fast_store_kind(A.out_type, A.mode);
   // The FP32 trial data are read with ordinary (vector) loads so that they live in
   // VGPRs: gfx950 dual-issues FP32 VALU operations only without SGPR operands.
   const float4* trial_pts32 = (const float4*)A.trial_pts32;
@@ -331,7 +332,7 @@ __device__ __forceinline__ void p0_tile_mixed(Args& A, const int2 tile, int symm
         acc = {(double)af.re, (double)af.im};
       }
       const cplx v = {acc.re * kInv4Pi, acc.im * kInv4Pi};
-      if (i_ok) store_value(A.out, (long long)i + A.ld * (long long)j, v, A.out_type, A.mode);
+      if (i_ok) store_entry(A.out, (long long)i + A.ld * (long long)j, v, skind, A.out_type, A.mode);
       if (mirror) lds[wave][jj][lane] = make_double2(v.re, v.im);
     }
     if (mirror) {
@@ -344,7 +345,7 @@ __device__ __forceinline__ void p0_tile_mixed(Args& A, const int2 tile, int symm
         const int ic = tile.x * kTile + col;
         const double2 val = lds[wave][row][col];
         if (jr < A.n_trial && ic < A.n_test)
-          store_value(A.out, (long long)jr + A.ld * (long long)ic, {val.x, val.y}, A.out_type, A.mode);
+          store_entry(A.out, (long long)jr + A.ld * (long long)ic, {val.x, val.y}, skind, A.out_type, A.mode);
       }
       __syncthreads();
     }
