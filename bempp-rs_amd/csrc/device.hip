@@ -39,6 +39,8 @@ struct DeviceGridCache {
   int nnear = -1;                 // sorted by (test block, trial block) of 64 cells
   int4* near_segs = nullptr;      // per block pair with near pairs: test block, trial block, first, count
   int nsegs = 0;
+  unsigned char* far_tiles = nullptr;  // [nblocks][nblocks]: every cell pair of the tile is far
+  int far_tiles_ld = 0;
 };
 
 struct DeviceSpaceCache {
@@ -61,6 +63,7 @@ void free_device_grid_cache(DeviceGridCache* c) {
     for (auto& kv : c->pts32) (void)hipFree(kv.second);
     (void)hipFree(c->near);
     (void)hipFree(c->near_segs);
+    (void)hipFree(c->far_tiles);
     (void)hipFree(c->pairs);
     (void)hipSetDevice(cur);
   }
@@ -309,6 +312,52 @@ const int2* grid_near_pairs(Grid& g, DeviceGridCache* c, int* n, const int4** se
   if (segs) *segs = c->near_segs;
   if (nsegs) *nsegs = c->nsegs;
   return c->near;
+}
+
+// Tiles (64-cell blocks a, b) whose cell pairs are all far for the mixed-precision split:
+// with block centre C (mean centroid), rho (largest centroid distance from C) and rmax
+// (largest cell radius), every pair has R >= |C_a - C_b| - rho_a - rho_b; the tile is
+// flagged when that bound exceeds the far limit kNearFactor (rmax_a + rmax_b) by 0.1 %, far
+// beyond the rounding of the per-pair test it replaces (no flagged pair can be near or
+// adjacent, adjacent cells being closer than rad_i + rad_j).
+const unsigned char* grid_far_tiles(Grid& g, DeviceGridCache* c, int* ld) {
+  std::lock_guard<std::mutex> lk(g.cache_mutex);
+  if (!c->far_tiles) {
+    const int64_t nc = g.nc();
+    const int nb = (int)((nc + dev::kTile - 1) / dev::kTile);
+    std::vector<double> cen(3 * nc), rad(nc), bc(3 * nb, 0.0), rho(nb, 0.0), rmax(nb, 0.0);
+    std::vector<double> rec(dev::kGeoStride);
+    for (int64_t i = 0; i < nc; ++i) {
+      cell_record(g, i, rec.data());
+      for (int d = 0; d < 3; ++d) cen[3 * i + d] = rec[dev::kGeoCentroid + d];
+      rad[i] = rec[dev::kGeoRadius];
+      const int b = (int)(i / dev::kTile);
+      for (int d = 0; d < 3; ++d) bc[3 * b + d] += cen[3 * i + d];
+      rmax[b] = std::max(rmax[b], rad[i]);
+    }
+    for (int b = 0; b < nb; ++b) {
+      const int64_t n = std::min<int64_t>(dev::kTile, nc - (int64_t)b * dev::kTile);
+      for (int d = 0; d < 3; ++d) bc[3 * b + d] /= (double)n;
+    }
+    for (int64_t i = 0; i < nc; ++i) {
+      const int b = (int)(i / dev::kTile);
+      double s = 0.0;
+      for (int d = 0; d < 3; ++d) s += (cen[3 * i + d] - bc[3 * b + d]) * (cen[3 * i + d] - bc[3 * b + d]);
+      rho[b] = std::max(rho[b], std::sqrt(s));
+    }
+    std::vector<unsigned char> flags((size_t)nb * nb, 0);
+    for (int a = 0; a < nb; ++a)
+      for (int b = 0; b < nb; ++b) {
+        double s = 0.0;
+        for (int d = 0; d < 3; ++d) s += (bc[3 * a + d] - bc[3 * b + d]) * (bc[3 * a + d] - bc[3 * b + d]);
+        const double lower = std::sqrt(s) - rho[a] - rho[b];
+        flags[(size_t)a * nb + b] = lower >= dev::kNearFactor * 1.001 * (rmax[a] + rmax[b]) ? 1 : 0;
+      }
+    c->far_tiles = upload(flags);
+    c->far_tiles_ld = nb;
+  }
+  *ld = c->far_tiles_ld;
+  return c->far_tiles;
 }
 
 const int4* grid_pairs(Grid& g, DeviceGridCache* c, int* npairs) {
@@ -650,12 +699,15 @@ Prepared prepare(const Assembler& a, const FunctionSpace& trial_c, const Functio
   P.near_segs = nullptr;
   P.nsegs = 0;
   R.test_pts32 = R.trial_pts32 = nullptr;
+  R.far_tiles = nullptr;
+  R.far_tiles_ld = 0;
   if (P.mixed) {
     // the mixed P0 tiles sum point pairs per weight class (kernels_mixed.hpp)
     for (int p = 0; p < P.Q; ++p)
       if (w[p] != w[dev::class_point(P.Q, dev::weight_class(P.Q, p))])
         throw Error("mixed precision: quadrature rule weights do not follow the orbit classes");
     R.test_pts32 = R.trial_pts32 = grid_points32_for_rule(tgrid, tc, P.Q, xi, w);
+    R.far_tiles = grid_far_tiles(tgrid, tc, &R.far_tiles_ld);
     P.near = grid_near_pairs(tgrid, tc, &P.nnear, &P.near_segs, &P.nsegs);
   }
   R.test_geo = tc->geo;

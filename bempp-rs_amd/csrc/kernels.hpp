@@ -64,6 +64,10 @@ struct RegularArgs {
   // rule weight x basis value per side (a P0 side: {w, 0, 0}), FP32 (mixed-precision P1 tiles)
   float wphi_test[kMaxQ][3];
   float wphi_trial[kMaxQ][3];
+  // mixed precision, one grid: far_tiles[a * far_tiles_ld + b] != 0 when every cell pair of tile
+  // (a, b) is far (then no pair of it is adjacent or near); null when not computed
+  const unsigned char* far_tiles;
+  int far_tiles_ld;
   void* out;
   long long ld;
   int out_type;

@@ -219,6 +219,8 @@ __device__ __forceinline__ void p0_tile_mixed(Args& A, const int2 tile, int symm
   const int tv0 = A.test_vid[4 * il], tv1 = A.test_vid[4 * il + 1], tv2 = A.test_vid[4 * il + 2];
   const bool mirror = symmetric && tile.x != tile.y;
   const int skind = fast_store_kind(A.out_type, A.mode);
+  // every cell pair of the tile far (host bound): no adjacency or near test per pair
+  const bool tile_far = A.far_tiles != nullptr && A.far_tiles[(size_t)tile.x * A.far_tiles_ld + tile.y] != 0;
   // The FP32 trial data are read with ordinary (vector) loads so that they live in
   // VGPRs: gfx950 dual-issues FP32 VALU operations only without SGPR operands.
   const float4* trial_pts32 = (const float4*)A.trial_pts32;
@@ -247,16 +249,22 @@ __device__ __forceinline__ void p0_tile_mixed(Args& A, const int2 tile, int symm
       const double cs[3] = {sg[kGeoCentroid], sg[kGeoCentroid + 1], sg[kGeoCentroid + 2]};
       const double srad = sg[kGeoRadius];
       const float ny[3] = {(float)sg[9], (float)sg[10], (float)sg[11]};
-      const int sv0 = trial_vid[4 * j], sv1 = trial_vid[4 * j + 1], sv2 = trial_vid[4 * j + 2];
-      const bool adjacent = A.same_grid && (tv0 == sv0 || tv0 == sv1 || tv0 == sv2 || tv1 == sv0 || tv1 == sv1 ||
-                                            tv1 == sv2 || tv2 == sv0 || tv2 == sv1 || tv2 == sv2);
       // cell-pair quantities in FP64
-      double R2;
-      const bool far = far_pair(ct, cs, trad, srad, &R2) && !adjacent;
       const double Dx = ct[0] - cs[0], Dy = ct[1] - cs[1], Dz = ct[2] - cs[2];
+      double R2;
+      bool far;
+      if (tile_far) {
+        R2 = fma(Dx, Dx, fma(Dy, Dy, Dz * Dz));
+        far = true;
+      } else {
+        const int sv0 = trial_vid[4 * j], sv1 = trial_vid[4 * j + 1], sv2 = trial_vid[4 * j + 2];
+        const bool adjacent = A.same_grid && (tv0 == sv0 || tv0 == sv1 || tv0 == sv2 || tv1 == sv0 ||
+                                              tv1 == sv1 || tv1 == sv2 || tv2 == sv0 || tv2 == sv1 || tv2 == sv2);
+        far = far_pair(ct, cs, trad, srad, &R2) && !adjacent;
+      }
       cplx acc = {0.0, 0.0};
       if (far) {
-        const double R = R2 * rsqrt_nr(R2);
+        const double R = R2 * rsqrt_fast(R2);
         const float D[3] = {(float)Dx, (float)Dy, (float)Dz};
         const float D2[3] = {2.0f * D[0], 2.0f * D[1], 2.0f * D[2]};
         const float R2f = (float)(HELM ? kappa * kappa * R2 : R2), Rf = (float)(kappa * R);
