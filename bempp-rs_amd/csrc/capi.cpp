@@ -29,7 +29,7 @@ struct SpaceT {
   std::shared_ptr<FunctionSpace> s;
   const GridT* grid = nullptr;
   uint8_t dtype = BEMPP_DTYPE_F64;
-  mutable std::vector<int64_t> point_dof;  // P1 continuous: point -> DOF (built on first use)
+  std::vector<int64_t> point_dof;  // continuous P1: point -> DOF (built with the handle)
 };
 struct ElementFamilyT {
   size_t degree;
@@ -282,6 +282,11 @@ SpaceT* function_space_new(const GridT* g, size_t degree, uint8_t continuity) {
         if (continuity > 1) throw Error("continuity must be 0 (standard) or 1 (discontinuous)");
         auto* sp = new SpaceT{std::make_shared<FunctionSpace>(g->g, (int)degree, continuity == BEMPP_CONTINUITY_STANDARD)};
         sp->grid = g;
+        const FunctionSpace& fs = *sp->s;
+        if (fs.continuous && fs.degree == 1) {
+          sp->point_dof.assign(fs.grid->nv(), -1);
+          for (size_t i = 0; i < fs.dofs.size(); ++i) sp->point_dof[fs.grid->cells[i]] = fs.dofs[i];
+        }
         return sp;
       },
       (SpaceT*)nullptr);
@@ -301,7 +306,7 @@ SpaceT* space_local_space(const SpaceT* s) {
   return guarded(
       [&]() -> SpaceT* {
         need(s, "space");
-        return new SpaceT{s->s, s->grid, s->dtype, {}};
+        return new SpaceT{s->s, s->grid, s->dtype, s->point_dof};
       },
       (SpaceT*)nullptr);
 }
@@ -321,10 +326,6 @@ const int64_t* entity_dofs(const SpaceT* s, size_t dim, size_t index, size_t* n)
     if (s->grid && !s->grid->topo_vertex_point.empty()) {
       if (index >= s->grid->topo_vertex_point.size()) return nullptr;
       point = s->grid->topo_vertex_point[index];
-    }
-    if (s->point_dof.empty()) {
-      s->point_dof.assign(fs.grid->nv(), -1);
-      for (size_t i = 0; i < fs.dofs.size(); ++i) s->point_dof[fs.grid->cells[i]] = fs.dofs[i];
     }
     if (point < 0 || point >= (int64_t)s->point_dof.size() || s->point_dof[point] < 0) return nullptr;
     *n = 1;

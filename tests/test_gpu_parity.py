@@ -290,3 +290,25 @@ def test_mixed_precision_mixed_spaces(test, trial):
     err = _rel(A, R)
     print(f"[mixed-p0p1] {test} x {trial}: rel Frobenius err {err:.3e}", flush=True)
     assert 1e-13 < err <= 1e-6
+
+
+@pytest.mark.parametrize("op", ["single_layer", "double_layer"])
+def test_mixed_precision_shuffled_cells(op):
+    """Mixed precision on a level-4 icosphere whose cells are randomly renumbered: 64-cell
+    blocks are then scattered over the sphere, so the per-tile all-far bound flags few tiles
+    and most pairs take the per-pair adjacency and near tests; the sorted order flags most
+    tiles. Both must give the same matrix (up to the renumbering) within the mixed tolerance,
+    and match the FP64 oracle."""
+    g0 = icosphere(4)
+    perm = np.random.default_rng(11).permutation(len(g0.cells))
+    cells = np.asarray(g0.cells)[perm]
+    g = Grid(g0.points, cells)
+    s, s0 = _space(g, "dp0"), _space(g0, "dp0")
+    a = _assembler("helmholtz", op, 2.5, _mixed_options())
+    A = a.assemble(s, s)
+    A0 = a.assemble(s0, s0)[np.ix_(perm, perm)]
+    R = O.assemble("helmholtz", op, g.points, cells, k=2.5)
+    err, err0 = _rel(A, R), _rel(A0, R)
+    print(f"[mixed-shuffled] {op}: shuffled {err:.3e}, sorted {err0:.3e}", flush=True)
+    assert 1e-12 < err <= 1e-6 and err0 <= 1e-6
+    assert np.abs(A - R).max() / np.abs(R).max() <= 1e-5
