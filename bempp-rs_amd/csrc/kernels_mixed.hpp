@@ -699,8 +699,12 @@ __global__ void __launch_bounds__(256) near_pair_kernel(const RegularArgs A, con
 // all items concatenated (item i owns near pairs [first[i], first[i + 1])), one
 // thread per pair.
 // ---------------------------------------------------------------------------
+// Waves per SIMD the persistent tile kernel asks for: 4 (128 VGPRs) where its registers fit
+// without spilling (SL/HS, 4-entry mirror segments), otherwise 3.
+constexpr int batched_mixed_waves(int op, int ch) { return (op == kSL || op == kHS) && ch == 4 ? 4 : 3; }
+
 template <int Q, int OP, bool HELM, int PH, int CH>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) batched_p0_mixed_persistent(const RegularArgs* __restrict__ items,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(batched_mixed_waves(OP, CH)))) batched_p0_mixed_persistent(const RegularArgs* __restrict__ items,
                                                                    const BatchTile* __restrict__ tiles, int ntiles,
                                                                    int* counter, int symmetric) {
   __shared__ double2 lds[4][CH][kTile];
