@@ -35,6 +35,7 @@ struct DeviceGridCache {
   std::vector<AdjacentPair> host_pairs;
   double max_radius = 0.0;        // largest centroid-to-vertex distance of any cell
   std::map<int, float*> pts32;    // regular point count -> [nc][Q][4] FP32 offsets from the centroid, w*jdet
+  std::map<int, float*> vtx32;    // regular point count -> [nc][3][4] FP32 vertex offsets, class weights
   int2* near = nullptr;           // non-adjacent pairs closer than the mixed-precision far bound (superset),
   int nnear = -1;                 // sorted by (test block, trial block) of 64 cells
   int4* near_segs = nullptr;      // per block pair with near pairs: test block, trial block, first, count
@@ -61,6 +62,7 @@ void free_device_grid_cache(DeviceGridCache* c) {
     (void)hipFree(c->vid);
     for (auto& kv : c->pts) (void)hipFree(kv.second);
     for (auto& kv : c->pts32) (void)hipFree(kv.second);
+    for (auto& kv : c->vtx32) (void)hipFree(kv.second);
     (void)hipFree(c->near);
     (void)hipFree(c->near_segs);
     (void)hipFree(c->far_tiles);
@@ -229,6 +231,43 @@ const float* grid_points32_for_rule(Grid& g, DeviceGridCache* c, int Q, const st
   float* d = upload(p32);
   c->pts32[Q] = d;
   return d;
+}
+
+// Per cell, for the mixed P0 tiles (rule points in orbit form c + t (V_k - c)):
+// d_0 = V_0 - c, |d_0|^2; d_1 = V_1 - c, |d_1|^2; the class weights w * jdet, |d_2|^2.
+const float* grid_vertex32_for_rule(Grid& g, DeviceGridCache* c, int Q, const std::vector<double>& w) {
+  std::lock_guard<std::mutex> lk(g.cache_mutex);
+  auto it = c->vtx32.find(Q);
+  if (it != c->vtx32.end()) return it->second;
+  const int64_t nc = g.nc();
+  std::vector<float> v32(nc * 12, 0.0f);
+  for (int64_t i = 0; i < nc; ++i) {
+    const double* v[3] = {&g.points[3 * g.cells[3 * i]], &g.points[3 * g.cells[3 * i + 1]],
+                          &g.points[3 * g.cells[3 * i + 2]]};
+    double cen[3], d[3][3], J0[3], J1[3];
+    for (int k = 0; k < 3; ++k) {
+      cen[k] = (v[0][k] + v[1][k] + v[2][k]) / 3.0;
+      J0[k] = v[1][k] - v[0][k];
+      J1[k] = v[2][k] - v[0][k];
+    }
+    for (int m = 0; m < 3; ++m)
+      for (int k = 0; k < 3; ++k) d[m][k] = v[m][k] - cen[k];
+    const double cr[3] = {J0[1] * J1[2] - J0[2] * J1[1], J0[2] * J1[0] - J0[0] * J1[2],
+                          J0[0] * J1[1] - J0[1] * J1[0]};
+    const double jdet = std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+    float* o = &v32[i * 12];
+    for (int m = 0; m < 2; ++m)
+      for (int k = 0; k < 3; ++k) o[4 * m + k] = (float)d[m][k];
+    for (int m = 0; m < 3; ++m) {
+      const float g2 = (float)(d[m][0] * d[m][0] + d[m][1] * d[m][1] + d[m][2] * d[m][2]);
+      o[m == 2 ? 11 : 4 * m + 3] = g2;
+    }
+    for (int cl = 0; cl < dev::weight_classes(Q); ++cl)
+      o[8 + cl] = (float)(w[dev::class_point(Q, cl)] * jdet);
+  }
+  float* dptr = upload(v32);
+  c->vtx32[Q] = dptr;
+  return dptr;
 }
 
 // Non-adjacent cell pairs with centroid distance below 1.01 x the far bound
@@ -699,6 +738,7 @@ Prepared prepare(const Assembler& a, const FunctionSpace& trial_c, const Functio
   P.near_segs = nullptr;
   P.nsegs = 0;
   R.test_pts32 = R.trial_pts32 = nullptr;
+  R.trial_vtx32 = nullptr;
   R.far_tiles = nullptr;
   R.far_tiles_ld = 0;
   if (P.mixed) {
@@ -706,7 +746,17 @@ Prepared prepare(const Assembler& a, const FunctionSpace& trial_c, const Functio
     for (int p = 0; p < P.Q; ++p)
       if (w[p] != w[dev::class_point(P.Q, dev::weight_class(P.Q, p))])
         throw Error("mixed precision: quadrature rule weights do not follow the orbit classes");
+    // ... and take the trial points as c + t (V_k - c) with the tabulated t, k
+    for (int p = 0; p < P.Q; ++p) {
+      const double lam[3] = {1.0 - xi[2 * p] - xi[2 * p + 1], xi[2 * p], xi[2 * p + 1]};
+      const double t = dev::orbit_t(P.Q, p);
+      const int kv = dev::orbit_vertex(P.Q, p);
+      for (int m = 0; m < 3; ++m)
+        if (std::fabs(lam[m] - (m == kv ? (1.0 + 2.0 * t) / 3.0 : (1.0 - t) / 3.0)) > 2e-7)
+          throw Error("mixed precision: quadrature rule points are not in the tabulated orbit form");
+    }
     R.test_pts32 = R.trial_pts32 = grid_points32_for_rule(tgrid, tc, P.Q, xi, w);
+    R.trial_vtx32 = grid_vertex32_for_rule(tgrid, tc, P.Q, w);
     R.far_tiles = grid_far_tiles(tgrid, tc, &R.far_tiles_ld);
     P.near = grid_near_pairs(tgrid, tc, &P.nnear, &P.near_segs, &P.nsegs);
   }

@@ -40,6 +40,9 @@ struct RegularArgs {
   const double* trial_pts;  // [n_trial][Q][4]
   const float* test_pts32;  // [n_test][Q][4]: x - centroid (3), w_p * jdet, FP32 (mixed precision)
   const float* trial_pts32; // [n_trial][Q][4]
+  // [n_trial][3][4] FP32, mixed-precision P0 tiles: V_0 - c, |V_0 - c|^2; V_1 - c, |V_1 - c|^2;
+  // w_class * jdet for weight classes 0..2, |V_2 - c|^2 (c the centroid, V_k the vertices)
+  const float* trial_vtx32;
   const double* test_geo;   // [n_test][kGeoStride]
   const double* trial_geo;  // [n_trial][kGeoStride]
   const int* test_vid;      // [n_test][4]

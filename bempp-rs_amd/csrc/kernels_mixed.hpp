@@ -27,13 +27,26 @@ constexpr double kNearFactor = 3.0;
 // FP32, unweighted, and weighted once per cell pair in FP64. Q = 1, 3: one class;
 // Q = 6: {0,1,2} {3,4,5}; Q = 7: {0} {1,2,3} {4,5,6}. The host checks a rule against
 // these classes before launching a kernel that relies on them (rule_has_weight_classes).
+__host__ __device__ constexpr int weight_classes(int Q) { return Q == 6 ? 2 : Q == 7 ? 3 : 1; }
 template <int Q>
-constexpr int kWeightClasses = Q == 6 ? 2 : Q == 7 ? 3 : 1;
+constexpr int kWeightClasses = weight_classes(Q);
 __host__ __device__ constexpr int weight_class(int Q, int p) {
   return Q == 6 ? (p >= 3 ? 1 : 0) : Q == 7 ? (p == 0 ? 0 : p <= 3 ? 1 : 2) : 0;
 }
 // first point of class c
 __host__ __device__ constexpr int class_point(int Q, int c) { return Q == 7 ? (c == 0 ? 0 : 3 * c - 2) : 3 * c; }
+
+// Orbit form of the same rules: point q is c + t (V_k - c) (c the centroid, V_k a vertex).
+// An S21 orbit with repeated barycentric coordinate a has t = 1 - 3a and its three points
+// take k = 0, 1, 2 in order; the centroid has t = 0. The host checks a rule against this
+// table before launching the mixed P0 tiles, which take the trial points in this form.
+__host__ __device__ constexpr float orbit_t(int Q, int q) {
+  return Q == 3   ? 0.5f
+         : Q == 6 ? (q < 3 ? -0.33784547274789467f : 0.7252713594706878f)
+         : Q == 7 ? (q == 0 ? 0.0f : q <= 3 ? 0.696140478029631f : -0.4104261923153453f)
+                  : 0.0f;
+}
+__host__ __device__ constexpr int orbit_vertex(int Q, int q) { return Q == 7 ? (q == 0 ? 0 : (q - 1) % 3) : q % 3; }
 
 // Far-pair predicate shared by the mixed tile kernel and the near-pair kernel.
 // Written with explicit fma so both evaluate it bitwise identically: every
@@ -223,7 +236,7 @@ __device__ __forceinline__ void p0_tile_mixed(Args& A, const int2 tile, int symm
   const bool tile_far = A.far_tiles != nullptr && A.far_tiles[(size_t)tile.x * A.far_tiles_ld + tile.y] != 0;
   // The FP32 trial data are read with ordinary (vector) loads so that they live in
   // VGPRs: gfx950 dual-issues FP32 VALU operations only without SGPR operands.
-  const float4* trial_pts32 = (const float4*)A.trial_pts32;
+  const float4* trial_vtx32 = (const float4*)A.trial_vtx32;
   const auto trial_geo = constant_view(A.trial_geo);
   const auto trial_vid = constant_view(A.trial_vid);
   static_assert(CH == 4 || CH == 8 || CH == 16, "chunk must divide 16 and 64");
@@ -236,15 +249,12 @@ __device__ __forceinline__ void p0_tile_mixed(Args& A, const int2 tile, int symm
         lds[wave][jj][lane] = make_double2(0.0, 0.0);
         continue;
       }
-      float sp[Q * 4];
-#pragma unroll
-      for (int q = 0; q < Q; ++q) {
-        const float4 t = trial_pts32[(size_t)j * Q + q];
-        sp[4 * q] = t.x;
-        sp[4 * q + 1] = t.y;
-        sp[4 * q + 2] = t.z;
-        sp[4 * q + 3] = t.w;
-      }
+      // trial vertex offsets d_0, d_1 (d_2 = -d_0 - d_1), |d_k|^2, class weights
+      const float4 ve0 = trial_vtx32[3 * (size_t)j], ve1 = trial_vtx32[3 * (size_t)j + 1],
+                   ve2 = trial_vtx32[3 * (size_t)j + 2];
+      const float dk[2][3] = {{ve0.x, ve0.y, ve0.z}, {ve1.x, ve1.y, ve1.z}};
+      const float gk[3] = {ve0.w, ve1.w, ve2.w};
+      const float wcl[3] = {ve2.x, ve2.y, ve2.z};
       const auto sg = trial_geo + (size_t)j * kGeoStride;
       const double cs[3] = {sg[kGeoCentroid], sg[kGeoCentroid + 1], sg[kGeoCentroid + 2]};
       const double srad = sg[kGeoRadius];
@@ -278,12 +288,24 @@ __device__ __forceinline__ void p0_tile_mixed(Args& A, const int2 tile, int symm
         if constexpr (OP == 1) Dn64 = Dx * sg[9] + Dy * sg[10] + Dz * sg[11];
         if constexpr (OP == 2) Dn64 = -(Dx * tgeo[9] + Dy * tgeo[10] + Dz * tgeo[11]);
         const float Dn = (float)Dn64;
-        float bq[Q], nq[Q];
+        // Trial points o_q = t_q d_k(q) (orbit form): with w_pk = (m_p - 2 sigma D).d_k,
+        //   sigma s_pq = sigma a_p + sigma t_q^2 |d_k|^2 + t_q w_pk,
+        // and w_p2 = -w_p0 - w_p1 since the d_k sum to zero.
+        const float Dm[3] = {-sig * D2[0], -sig * D2[1], -sig * D2[2]};
+        const float Dd[2] = {fmaf(Dm[0], dk[0][0], fmaf(Dm[1], dk[0][1], Dm[2] * dk[0][2])),
+                             fmaf(Dm[0], dk[1][0], fmaf(Dm[1], dk[1][1], Dm[2] * dk[1][2]))};
+        float cq[Q], nq[Q];  // sigma t_q^2 |d_k|^2 and (DL/ADL) o_q.n
+        float dkn[3] = {0.0f, 0.0f, 0.0f};
+        if constexpr (kDN) {
+          dkn[0] = sgn * fmaf(dk[0][0], nrm[0], fmaf(dk[0][1], nrm[1], dk[0][2] * nrm[2]));
+          dkn[1] = sgn * fmaf(dk[1][0], nrm[0], fmaf(dk[1][1], nrm[1], dk[1][2] * nrm[2]));
+          dkn[2] = -(dkn[0] + dkn[1]);
+        }
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-          const float ox = sp[4 * q], oy = sp[4 * q + 1], oz = sp[4 * q + 2];
-          bq[q] = sig * fmaf(D2[0] - ox, ox, fmaf(D2[1] - oy, oy, (D2[2] - oz) * oz));
-          if constexpr (kDN) nq[q] = sgn * fmaf(ox, nrm[0], fmaf(oy, nrm[1], oz * nrm[2]));
+          const float t = orbit_t(Q, q);
+          cq[q] = t == 0.0f ? 0.0f : (sig * (t * t)) * gk[orbit_vertex(Q, q)];
+          if constexpr (kDN) nq[q] = t * dkn[orbit_vertex(Q, q)];
         }
         cplxf cls[NC][NC];  // unweighted sums per (test class, trial class)
 #pragma unroll
@@ -297,9 +319,13 @@ __device__ __forceinline__ void p0_tile_mixed(Args& A, const int2 tile, int symm
           const float ap = fmaf(-D[0], mx, fmaf(-D[1], my, fmaf(-D[2], mz, oo[p])));
           float dnp = 0.0f;
           if constexpr (kDN) dnp = fmaf(mx, hn[0], fmaf(my, hn[1], fmaf(mz, hn[2], Dn)));
+          const float w0 = fmaf(mx, dk[0][0], fmaf(my, dk[0][1], fmaf(mz, dk[0][2], Dd[0])));
+          const float w1 = fmaf(mx, dk[1][0], fmaf(my, dk[1][1], fmaf(mz, dk[1][2], Dd[1])));
+          const float wk[3] = {w0, w1, -(w0 + w1)};
 #pragma unroll
           for (int q = 0; q < Q; ++q) {
-            const float s = fmaf(mx, sp[4 * q], fmaf(my, sp[4 * q + 1], fmaf(mz, sp[4 * q + 2], ap - bq[q])));
+            const float t = orbit_t(Q, q);
+            const float s = t == 0.0f ? ap : fmaf(t, wk[orbit_vertex(Q, q)], ap + cq[q]);
             cplxf& cl = cls[weight_class(Q, p)][weight_class(Q, q)];
             if constexpr (HELM)
               far_accumulate_scaled<OP, PH>(s, kDN ? dnp - nq[q] : 0.0f, R2f, Rf, cl);
@@ -315,7 +341,7 @@ __device__ __forceinline__ void p0_tile_mixed(Args& A, const int2 tile, int symm
           cplxf row = {0.0f, 0.0f};
 #pragma unroll
           for (int b = 0; b < NC; ++b) {
-            const float ws = sp[4 * class_point(Q, b) + 3];
+            const float ws = wcl[b];
             row.re = fmaf(cls[a][b].re, ws, row.re);
             row.im = fmaf(cls[a][b].im, ws, row.im);
           }

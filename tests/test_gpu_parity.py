@@ -248,6 +248,25 @@ def test_mixed_precision_wide_phase(op):
     assert err > 1e-12
 
 
+@pytest.mark.parametrize("nq", [1, 3, 7])
+@pytest.mark.parametrize("op", list(OPS))
+def test_mixed_precision_rules(nq, op):
+    """Mixed-precision P0 tiles with the 1-, 3- and 7-point regular rules, whose trial points
+    the tiles take in orbit form c + t (V_k - c) (centroid only; one orbit; centroid and two
+    orbits): Helmholtz k = 2.5 on the level-3 icosphere, <= 1e-6 against the FP64 oracle
+    with the same rule."""
+    g = icosphere(3)
+    s = _space(g, "dp0")
+    a = _assembler("helmholtz", op, 2.5, _mixed_options(nq))
+    print(f"[mixed-rule] {op} nq={nq}", flush=True)
+    A = a.assemble(s, s)
+    R = O.assemble("helmholtz", op, g.points, g.cells, k=2.5, nq=nq)
+    err = _rel(A, R)
+    emax = np.abs(A - R).max() / np.abs(R).max()
+    print(f"[mixed-rule]   rel Frobenius err {err:.3e}, max entry err {emax:.3e}", flush=True)
+    assert 1e-13 < err <= 1e-6 and emax <= 1e-5
+
+
 @pytest.mark.parametrize("op", ["single_layer", "double_layer"])
 def test_mixed_precision_complex64_output(op):
     """The complex64-mixed form: FP32 far field, FP64 near/adjacent pairs and accumulation,
