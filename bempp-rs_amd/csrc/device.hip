@@ -36,6 +36,7 @@ struct DeviceGridCache {
   double max_radius = 0.0;        // largest centroid-to-vertex distance of any cell
   std::map<int, float*> pts32;    // regular point count -> [nc][Q][4] FP32 offsets from the centroid, w*jdet
   std::map<int, float*> vtx32;    // regular point count -> [nc][3][4] FP32 vertex offsets, class weights
+  std::map<int, double*> vtx64;   // the same in FP64
   int2* near = nullptr;           // non-adjacent pairs closer than the mixed-precision far bound (superset),
   int nnear = -1;                 // sorted by (test block, trial block) of 64 cells
   int4* near_segs = nullptr;      // per block pair with near pairs: test block, trial block, first, count
@@ -63,6 +64,7 @@ void free_device_grid_cache(DeviceGridCache* c) {
     for (auto& kv : c->pts) (void)hipFree(kv.second);
     for (auto& kv : c->pts32) (void)hipFree(kv.second);
     for (auto& kv : c->vtx32) (void)hipFree(kv.second);
+    for (auto& kv : c->vtx64) (void)hipFree(kv.second);
     (void)hipFree(c->near);
     (void)hipFree(c->near_segs);
     (void)hipFree(c->far_tiles);
@@ -233,14 +235,12 @@ const float* grid_points32_for_rule(Grid& g, DeviceGridCache* c, int Q, const st
   return d;
 }
 
-// Per cell, for the mixed P0 tiles (rule points in orbit form c + t (V_k - c)):
-// d_0 = V_0 - c, |d_0|^2; d_1 = V_1 - c, |d_1|^2; the class weights w * jdet, |d_2|^2.
-const float* grid_vertex32_for_rule(Grid& g, DeviceGridCache* c, int Q, const std::vector<double>& w) {
-  std::lock_guard<std::mutex> lk(g.cache_mutex);
-  auto it = c->vtx32.find(Q);
-  if (it != c->vtx32.end()) return it->second;
+// Per cell, for the P0 tiles (rule points in orbit form c + t (V_k - c)): d_0 = V_0 - c,
+// |d_0|^2; d_1 = V_1 - c, |d_1|^2; the class weights w * jdet, |d_2|^2 (12 values of type T).
+template <typename T>
+std::vector<T> vertex_records(const Grid& g, int Q, const std::vector<double>& w) {
   const int64_t nc = g.nc();
-  std::vector<float> v32(nc * 12, 0.0f);
+  std::vector<T> out(nc * 12, T(0));
   for (int64_t i = 0; i < nc; ++i) {
     const double* v[3] = {&g.points[3 * g.cells[3 * i]], &g.points[3 * g.cells[3 * i + 1]],
                           &g.points[3 * g.cells[3 * i + 2]]};
@@ -255,19 +255,46 @@ const float* grid_vertex32_for_rule(Grid& g, DeviceGridCache* c, int Q, const st
     const double cr[3] = {J0[1] * J1[2] - J0[2] * J1[1], J0[2] * J1[0] - J0[0] * J1[2],
                           J0[0] * J1[1] - J0[1] * J1[0]};
     const double jdet = std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
-    float* o = &v32[i * 12];
+    T* o = &out[i * 12];
     for (int m = 0; m < 2; ++m)
-      for (int k = 0; k < 3; ++k) o[4 * m + k] = (float)d[m][k];
-    for (int m = 0; m < 3; ++m) {
-      const float g2 = (float)(d[m][0] * d[m][0] + d[m][1] * d[m][1] + d[m][2] * d[m][2]);
-      o[m == 2 ? 11 : 4 * m + 3] = g2;
-    }
-    for (int cl = 0; cl < dev::weight_classes(Q); ++cl)
-      o[8 + cl] = (float)(w[dev::class_point(Q, cl)] * jdet);
+      for (int k = 0; k < 3; ++k) o[4 * m + k] = (T)d[m][k];
+    for (int m = 0; m < 3; ++m)
+      o[m == 2 ? 11 : 4 * m + 3] = (T)(d[m][0] * d[m][0] + d[m][1] * d[m][1] + d[m][2] * d[m][2]);
+    for (int cl = 0; cl < dev::weight_classes(Q); ++cl) o[8 + cl] = (T)(w[dev::class_point(Q, cl)] * jdet);
   }
-  float* dptr = upload(v32);
-  c->vtx32[Q] = dptr;
-  return dptr;
+  return out;
+}
+
+const float* grid_vertex32_for_rule(Grid& g, DeviceGridCache* c, int Q, const std::vector<double>& w) {
+  std::lock_guard<std::mutex> lk(g.cache_mutex);
+  auto it = c->vtx32.find(Q);
+  if (it != c->vtx32.end()) return it->second;
+  float* d = upload(vertex_records<float>(g, Q, w));
+  c->vtx32[Q] = d;
+  return d;
+}
+
+const double* grid_vertex64_for_rule(Grid& g, DeviceGridCache* c, int Q, const std::vector<double>& w) {
+  std::lock_guard<std::mutex> lk(g.cache_mutex);
+  auto it = c->vtx64.find(Q);
+  if (it != c->vtx64.end()) return it->second;
+  double* d = upload(vertex_records<double>(g, Q, w));
+  c->vtx64[Q] = d;
+  return d;
+}
+
+// True when every point of the rule is c + t (V_k - c) with the tabulated t, k (orbit_t,
+// orbit_vertex) and the weights follow the weight classes.
+bool rule_in_orbit_form(int Q, const std::vector<double>& xi, const std::vector<double>& w) {
+  for (int p = 0; p < Q; ++p) {
+    if (w[p] != w[dev::class_point(Q, dev::weight_class(Q, p))]) return false;
+    const double lam[3] = {1.0 - xi[2 * p] - xi[2 * p + 1], xi[2 * p], xi[2 * p + 1]};
+    const double t = dev::orbit_t(Q, p);
+    const int kv = dev::orbit_vertex(Q, p);
+    for (int m = 0; m < 3; ++m)
+      if (std::fabs(lam[m] - (m == kv ? (1.0 + 2.0 * t) / 3.0 : (1.0 - t) / 3.0)) > 1e-15) return false;
+  }
+  return true;
 }
 
 // Non-adjacent cell pairs with centroid distance below 1.01 x the far bound
@@ -731,6 +758,10 @@ Prepared prepare(const Assembler& a, const FunctionSpace& trial_c, const Functio
   dev::RegularArgs& R = P.R;
   R.test_pts = grid_points_for_rule(tgrid, tc, P.Q, xi, w);
   R.trial_pts = grid_points_for_rule(sgrid, sc, P.Q, xi, w);
+  // FP64 P0 single-layer/hypersingular tiles take both rules' points in orbit form
+  R.orbit_form = rule_in_orbit_form(P.Q, xi, w) ? 1 : 0;
+  R.test_vtx64 = R.orbit_form ? grid_vertex64_for_rule(tgrid, tc, P.Q, w) : nullptr;
+  R.trial_vtx64 = R.orbit_form ? grid_vertex64_for_rule(sgrid, sc, P.Q, w) : nullptr;
   // mixed precision: one grid (P0 x P0: p0 mixed tiles; a P1 side: LDS mixed tiles)
   P.mixed = a.options.precision == kMixed && P.same_grid;
   P.near = nullptr;
@@ -742,19 +773,10 @@ Prepared prepare(const Assembler& a, const FunctionSpace& trial_c, const Functio
   R.far_tiles = nullptr;
   R.far_tiles_ld = 0;
   if (P.mixed) {
-    // the mixed P0 tiles sum point pairs per weight class (kernels_mixed.hpp)
-    for (int p = 0; p < P.Q; ++p)
-      if (w[p] != w[dev::class_point(P.Q, dev::weight_class(P.Q, p))])
-        throw Error("mixed precision: quadrature rule weights do not follow the orbit classes");
-    // ... and take the trial points as c + t (V_k - c) with the tabulated t, k
-    for (int p = 0; p < P.Q; ++p) {
-      const double lam[3] = {1.0 - xi[2 * p] - xi[2 * p + 1], xi[2 * p], xi[2 * p + 1]};
-      const double t = dev::orbit_t(P.Q, p);
-      const int kv = dev::orbit_vertex(P.Q, p);
-      for (int m = 0; m < 3; ++m)
-        if (std::fabs(lam[m] - (m == kv ? (1.0 + 2.0 * t) / 3.0 : (1.0 - t) / 3.0)) > 2e-7)
-          throw Error("mixed precision: quadrature rule points are not in the tabulated orbit form");
-    }
+    // the mixed P0 tiles sum point pairs per weight class and take the trial points in
+    // orbit form (kernels_mixed.hpp)
+    if (!rule_in_orbit_form(P.Q, xi, w))
+      throw Error("mixed precision: quadrature rule not in the tabulated orbit form");
     R.test_pts32 = R.trial_pts32 = grid_points32_for_rule(tgrid, tc, P.Q, xi, w);
     R.trial_vtx32 = grid_vertex32_for_rule(tgrid, tc, P.Q, w);
     R.far_tiles = grid_far_tiles(tgrid, tc, &R.far_tiles_ld);

@@ -43,6 +43,10 @@ struct RegularArgs {
   // [n_trial][3][4] FP32, mixed-precision P0 tiles: V_0 - c, |V_0 - c|^2; V_1 - c, |V_1 - c|^2;
   // w_class * jdet for weight classes 0..2, |V_2 - c|^2 (c the centroid, V_k the vertices)
   const float* trial_vtx32;
+  // [n][12] FP64, the same layout (FP64 P0 single-layer/hypersingular tiles when orbit_form)
+  const double* test_vtx64;
+  const double* trial_vtx64;
+  int orbit_form;
   const double* test_geo;   // [n_test][kGeoStride]
   const double* trial_geo;  // [n_trial][kGeoStride]
   const int* test_vid;      // [n_test][4]
@@ -147,9 +151,7 @@ __device__ __forceinline__ cplx kernel_scalar(double dx, double dy, double dz, d
 // acc += w G for the single-layer kernel (without 1/(4 pi)); the weight is folded into
 // 1/r so that the complex update costs one product and two fused multiply-adds.
 template <bool HELM, int CENTRED>
-__device__ __forceinline__ void green_accumulate(double dx, double dy, double dz, double k, double kR, double w,
-                                                 cplx& acc) {
-  const double r2 = dx * dx + dy * dy + dz * dz;
+__device__ __forceinline__ void green_accumulate_r2(double r2, double k, double kR, double w, cplx& acc) {
   const double rinv = rsqrt_fast(r2);
   if constexpr (HELM) {
     const double r = r2 * rinv;
@@ -164,6 +166,12 @@ __device__ __forceinline__ void green_accumulate(double dx, double dy, double dz
   } else {
     acc.re = fma(rinv, w, acc.re);
   }
+}
+
+template <bool HELM, int CENTRED>
+__device__ __forceinline__ void green_accumulate(double dx, double dy, double dz, double k, double kR, double w,
+                                                 cplx& acc) {
+  green_accumulate_r2<HELM, CENTRED>(dx * dx + dy * dy + dz * dz, k, kR, w, acc);
 }
 
 // Overwriting stores of the two complex output types take a short path; every other
@@ -559,6 +567,32 @@ __global__ void __launch_bounds__(256) regular_tiled_lds(const RegularArgs A, co
   flush_tile(A, acc, t0, ndt, s0, nds, symmetric && tile.x != tile.y);
 }
 
+// Weight classes of the symmetric triangle rules (triangle_rule): the points of one orbit
+// share a weight, so the mixed-precision far-field point-pair sums are taken per (test
+// class, trial class) in FP32, unweighted, and weighted once per cell pair. Q = 1, 3: one class;
+// Q = 6: {0,1,2} {3,4,5}; Q = 7: {0} {1,2,3} {4,5,6}. The host checks a rule against
+// these classes before launching a kernel that relies on them (rule_has_weight_classes).
+__host__ __device__ constexpr int weight_classes(int Q) { return Q == 6 ? 2 : Q == 7 ? 3 : 1; }
+template <int Q>
+constexpr int kWeightClasses = weight_classes(Q);
+__host__ __device__ constexpr int weight_class(int Q, int p) {
+  return Q == 6 ? (p >= 3 ? 1 : 0) : Q == 7 ? (p == 0 ? 0 : p <= 3 ? 1 : 2) : 0;
+}
+// first point of class c
+__host__ __device__ constexpr int class_point(int Q, int c) { return Q == 7 ? (c == 0 ? 0 : 3 * c - 2) : 3 * c; }
+
+// Orbit form of the same rules: point q is c + t (V_k - c) (c the centroid, V_k a vertex).
+// An S21 orbit with repeated barycentric coordinate a has t = 1 - 3a and its three points
+// take k = 0, 1, 2 in order; the centroid has t = 0. The host checks a rule against this
+// table before launching the P0 tiles, which take the rule points in this form.
+__host__ __device__ constexpr double orbit_t(int Q, int q) {
+  return Q == 3   ? 0.5
+         : Q == 6 ? (q < 3 ? -0.33784547274789466 : 0.72527135947068777)
+         : Q == 7 ? (q == 0 ? 0.0 : q <= 3 ? 0.69614047802963098 : -0.41042619231534527)
+                  : 0.0;
+}
+__host__ __device__ constexpr int orbit_vertex(int Q, int q) { return Q == 7 ? (q == 0 ? 0 : (q - 1) % 3) : q % 3; }
+
 // ---------------------------------------------------------------------------
 // P0 x P0 regular pairs, tiled. One workgroup (4 wavefronts) per 64 x 64 tile
 // (tile row a: test cells 64a + lane; tile column b: trial cells, 16 per
@@ -596,6 +630,61 @@ __device__ __forceinline__ cplx p0_pair(const double (&xp)[Q][3], const double (
   return acc;
 }
 
+// Single layer (and the P0 hypersingular n.n term) of one regular P0 cell pair with both
+// rules in orbit form, x_p = c_t + tau_p d'_k'(p), y_q = c_s + t_q d_k(q) (d', d the
+// vertex offsets of the test and trial cell from their centroids, D = c_t - c_s):
+//   r^2 = (R^2 + t_q^2 |d_k|^2) + A_p + t_q W_pk,
+//   A_p = tau_p 2 D.d'_k' + tau_p^2 |d'_k'|^2,  W_pk = -2 D.d_k - 2 tau_p d'_k'.d_k,
+// two instructions per point pair instead of six (three differences and a dot product);
+// W_p2 = -W_p0 - W_p1 and the d'_2 products follow the same way, the offsets of a cell
+// summing to zero. tau2g[p] = tau_p^2 |d'_k'(p)|^2; sv: the trial cell's 12-value record.
+template <int Q, bool HELM, int CENTRED, typename SV>
+__device__ __forceinline__ cplx p0_pair_orbit(const double (&td)[2][3], const double (&tau2g)[Q],
+                                              const double (&twc)[3], const SV& sv, const double (&D)[3], double R2,
+                                              double k, double kR) {
+  const double E0 = -2.0 * (D[0] * sv[0] + D[1] * sv[1] + D[2] * sv[2]);
+  const double E1 = -2.0 * (D[0] * sv[4] + D[1] * sv[5] + D[2] * sv[6]);
+  double N[3][2], F[3];
+#pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    N[m][0] = td[m][0] * sv[0] + td[m][1] * sv[1] + td[m][2] * sv[2];
+    N[m][1] = td[m][0] * sv[4] + td[m][1] * sv[5] + td[m][2] * sv[6];
+    F[m] = 2.0 * (D[0] * td[m][0] + D[1] * td[m][1] + D[2] * td[m][2]);
+  }
+  N[2][0] = -(N[0][0] + N[1][0]);
+  N[2][1] = -(N[0][1] + N[1][1]);
+  F[2] = -(F[0] + F[1]);
+  const double gk[3] = {sv[3], sv[7], sv[11]};
+  double RC[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const double t = orbit_t(Q, q);
+    RC[q] = t == 0.0 ? R2 : fma(t * t, gk[orbit_vertex(Q, q)], R2);
+  }
+  cplx acc = {0.0, 0.0};
+#pragma unroll
+  for (int p = 0; p < Q; ++p) {
+    const int kp = orbit_vertex(Q, p);
+    const double tau = orbit_t(Q, p);
+    double W[3];
+    W[0] = tau == 0.0 ? E0 : fma(-2.0 * tau, N[kp][0], E0);
+    W[1] = tau == 0.0 ? E1 : fma(-2.0 * tau, N[kp][1], E1);
+    W[2] = -(W[0] + W[1]);
+    const double Ap = fma(tau, F[kp], tau2g[p]);
+    cplx ap = {0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const double t = orbit_t(Q, q);
+      const double base = tau == 0.0 ? RC[q] : Ap + RC[q];
+      const double r2 = t == 0.0 ? base : fma(t, W[orbit_vertex(Q, q)], base);
+      green_accumulate_r2<HELM, CENTRED>(r2, k, kR, sv[8 + weight_class(Q, q)], ap);
+    }
+    acc.re = fma(ap.re, twc[weight_class(Q, p)], acc.re);
+    acc.im = fma(ap.im, twc[weight_class(Q, p)], acc.im);
+  }
+  return acc;
+}
+
 // CH: trial cells per chunk (a wavefront covers 16 trial cells in 16 / CH chunks);
 // the mirror transpose stores CH contiguous entries per segment.
 // Args: RegularArgs, possibly qualified with the constant address space (batched
@@ -608,14 +697,33 @@ __device__ __forceinline__ void p0_tile(Args& A, const int2 tile, int symmetric,
   const int i = tile.x * kTile + lane;
   const bool i_ok = i < A.n_test;
   const int il = i_ok ? i : A.n_test - 1;  // clamped: loads stay in bounds
-  double xp[Q][3], wp[Q];
+  // single layer / hypersingular: rule points in orbit form (p0_pair_orbit); otherwise
+  // the points themselves
+  constexpr bool kOrbit = OP == 0 || OP == 3;
+  constexpr int QX = kOrbit ? 1 : Q;
+  double xp[QX][3], wp[QX];
+  double td[2][3], tau2g[Q], twc[3];
+  if constexpr (kOrbit) {
+    const double* tv = A.test_vtx64 + (size_t)il * 12;
 #pragma unroll
-  for (int p = 0; p < Q; ++p) {
-    const double* s = A.test_pts + ((size_t)il * Q + p) * 4;
-    xp[p][0] = s[0];
-    xp[p][1] = s[1];
-    xp[p][2] = s[2];
-    wp[p] = s[3];
+    for (int m = 0; m < 2; ++m)
+      for (int d = 0; d < 3; ++d) td[m][d] = tv[4 * m + d];
+    const double tg[3] = {tv[3], tv[7], tv[11]};
+#pragma unroll
+    for (int p = 0; p < Q; ++p) {
+      const double tau = orbit_t(Q, p);
+      tau2g[p] = tau * tau * tg[orbit_vertex(Q, p)];
+    }
+    for (int c = 0; c < 3; ++c) twc[c] = tv[8 + c];
+  } else {
+#pragma unroll
+    for (int p = 0; p < Q; ++p) {
+      const double* s = A.test_pts + ((size_t)il * Q + p) * 4;
+      xp[p][0] = s[0];
+      xp[p][1] = s[1];
+      xp[p][2] = s[2];
+      wp[p] = s[3];
+    }
   }
   const double* tgeo = A.test_geo + (size_t)il * kGeoStride;
   const double nx[3] = {tgeo[9], tgeo[10], tgeo[11]};
@@ -637,10 +745,11 @@ __device__ __forceinline__ void p0_tile(Args& A, const int2 tile, int symmetric,
         continue;
       }
       // wave-uniform trial cell: loaded unconditionally (scalar loads) before any per-lane branch
-      const auto spp = trial_pts + (size_t)j * Q * 4;
-      double sp[Q * 4];
+      constexpr int NSP = kOrbit ? 12 : Q * 4;
+      const auto spp = kOrbit ? constant_view(A.trial_vtx64) + (size_t)j * 12 : trial_pts + (size_t)j * Q * 4;
+      double sp[NSP];
 #pragma unroll
-      for (int m = 0; m < Q * 4; ++m) sp[m] = spp[m];
+      for (int m = 0; m < NSP; ++m) sp[m] = spp[m];
       const auto sg = trial_geo + (size_t)j * kGeoStride;
       const int sv0 = trial_vid[4 * j], sv1 = trial_vid[4 * j + 1], sv2 = trial_vid[4 * j + 2];
       const bool adjacent = A.same_grid && (tv0 == sv0 || tv0 == sv1 || tv0 == sv2 || tv1 == sv0 || tv1 == sv1 ||
@@ -648,7 +757,18 @@ __device__ __forceinline__ void p0_tile(Args& A, const int2 tile, int symmetric,
       const double ny[3] = {sg[9], sg[10], sg[11]};
       cplx acc = {0.0, 0.0};
       if (!adjacent) {
-        if constexpr (CENTRED) {
+        if constexpr (kOrbit) {
+          const double cs[3] = {sg[kGeoCentroid], sg[kGeoCentroid + 1], sg[kGeoCentroid + 2]};
+          const double D[3] = {ct[0] - cs[0], ct[1] - cs[1], ct[2] - cs[2]};
+          const double R2 = D[0] * D[0] + D[1] * D[1] + D[2] * D[2];
+          if constexpr (CENTRED) {
+            double kR;
+            const cplx e = pair_phase(ct, cs, A.k, &kR);
+            acc = cmul(p0_pair_orbit<Q, HELM, CENTRED>(td, tau2g, twc, sp, D, R2, A.k, kR), e);
+          } else {
+            acc = p0_pair_orbit<Q, HELM, 0>(td, tau2g, twc, sp, D, R2, A.k, 0.0);
+          }
+        } else if constexpr (CENTRED) {
           const double cs[3] = {sg[kGeoCentroid], sg[kGeoCentroid + 1], sg[kGeoCentroid + 2]};
           double kR;
           const cplx e = pair_phase(ct, cs, A.k, &kR);

@@ -22,31 +22,6 @@ namespace dev {
 
 constexpr double kNearFactor = 3.0;
 
-// Weight classes of the symmetric triangle rules (triangle_rule): the points of one orbit
-// share a weight, so far-field point-pair sums are taken per (test class, trial class) in
-// FP32, unweighted, and weighted once per cell pair in FP64. Q = 1, 3: one class;
-// Q = 6: {0,1,2} {3,4,5}; Q = 7: {0} {1,2,3} {4,5,6}. The host checks a rule against
-// these classes before launching a kernel that relies on them (rule_has_weight_classes).
-__host__ __device__ constexpr int weight_classes(int Q) { return Q == 6 ? 2 : Q == 7 ? 3 : 1; }
-template <int Q>
-constexpr int kWeightClasses = weight_classes(Q);
-__host__ __device__ constexpr int weight_class(int Q, int p) {
-  return Q == 6 ? (p >= 3 ? 1 : 0) : Q == 7 ? (p == 0 ? 0 : p <= 3 ? 1 : 2) : 0;
-}
-// first point of class c
-__host__ __device__ constexpr int class_point(int Q, int c) { return Q == 7 ? (c == 0 ? 0 : 3 * c - 2) : 3 * c; }
-
-// Orbit form of the same rules: point q is c + t (V_k - c) (c the centroid, V_k a vertex).
-// An S21 orbit with repeated barycentric coordinate a has t = 1 - 3a and its three points
-// take k = 0, 1, 2 in order; the centroid has t = 0. The host checks a rule against this
-// table before launching the mixed P0 tiles, which take the trial points in this form.
-__host__ __device__ constexpr float orbit_t(int Q, int q) {
-  return Q == 3   ? 0.5f
-         : Q == 6 ? (q < 3 ? -0.33784547274789467f : 0.7252713594706878f)
-         : Q == 7 ? (q == 0 ? 0.0f : q <= 3 ? 0.696140478029631f : -0.4104261923153453f)
-                  : 0.0f;
-}
-__host__ __device__ constexpr int orbit_vertex(int Q, int q) { return Q == 7 ? (q == 0 ? 0 : (q - 1) % 3) : q % 3; }
 
 // Far-pair predicate shared by the mixed tile kernel and the near-pair kernel.
 // Written with explicit fma so both evaluate it bitwise identically: every
