@@ -758,10 +758,10 @@ Prepared prepare(const Assembler& a, const FunctionSpace& trial_c, const Functio
   dev::RegularArgs& R = P.R;
   R.test_pts = grid_points_for_rule(tgrid, tc, P.Q, xi, w);
   R.trial_pts = grid_points_for_rule(sgrid, sc, P.Q, xi, w);
-  // FP64 P0 single-layer/hypersingular tiles take both rules' points in orbit form
+  // the FP64 P0 x P0 tiles take both rules' points in orbit form
   R.orbit_form = rule_in_orbit_form(P.Q, xi, w) ? 1 : 0;
-  if (!R.orbit_form && test.nb == 1 && trial.nb == 1 && (a.op == kSL || a.op == kHS))
-    throw Error("P0 single-layer/hypersingular tiles: quadrature rule not in the tabulated orbit form");
+  if (!R.orbit_form && test.nb == 1 && trial.nb == 1)
+    throw Error("P0 tiles: quadrature rule not in the tabulated orbit form");
   R.test_vtx64 = R.orbit_form ? grid_vertex64_for_rule(tgrid, tc, P.Q, w) : nullptr;
   R.trial_vtx64 = R.orbit_form ? grid_vertex64_for_rule(sgrid, sc, P.Q, w) : nullptr;
   // mixed precision: one grid (P0 x P0: p0 mixed tiles; a P1 side: LDS mixed tiles)

@@ -43,7 +43,7 @@ struct RegularArgs {
   // [n_trial][3][4] FP32, mixed-precision P0 tiles: V_0 - c, |V_0 - c|^2; V_1 - c, |V_1 - c|^2;
   // w_class * jdet for weight classes 0..2, |V_2 - c|^2 (c the centroid, V_k the vertices)
   const float* trial_vtx32;
-  // [n][12] FP64, the same layout (FP64 P0 single-layer/hypersingular tiles when orbit_form)
+  // [n][12] FP64, the same layout (FP64 P0 x P0 tiles; orbit_form: the rule is in orbit form)
   const double* test_vtx64;
   const double* trial_vtx64;
   int orbit_form;
@@ -105,15 +105,13 @@ struct cplx {
   double re, im;
 };
 
-// Kernel scalar (without the 1/(4 pi) factor) for displacement d = x - y.
-// nx: test normal, ny: trial normal. CENTRED (Helmholtz, phase tier 1 or 2): the
-// phase is taken relative to kR, i.e. the result lacks the factor e^{ikR} that the
-// caller applies once per cell pair; requires |k r - kR| <= 0.5 (tier 1) or <= 0.25
-// (tier 2). CENTRED = 0: full argument reduction.
+// Kernel scalar (without the 1/(4 pi) factor) from r^2 = |x - y|^2 and, for the
+// double-layer kernels, dn = (x - y).n_y (DL) or -(x - y).n_x (ADL). CENTRED (Helmholtz,
+// phase tier 1 or 2): the phase is taken relative to kR, i.e. the result lacks the factor
+// e^{ikR} that the caller applies once per cell pair; requires |k r - kR| <= 0.5 (tier 1)
+// or <= 0.25 (tier 2). CENTRED = 0: full argument reduction.
 template <int OP, bool HELM, int CENTRED = 0>
-__device__ __forceinline__ cplx kernel_scalar(double dx, double dy, double dz, double k, const double* nx,
-                                              const double* ny, double kR = 0.0) {
-  const double r2 = dx * dx + dy * dy + dz * dz;
+__device__ __forceinline__ cplx kernel_scalar_r2(double r2, double dn, double k, double kR = 0.0) {
   const double rinv = rsqrt_fast(r2);
   cplx g;
   double kr = 0.0, s = 0.0, c = 1.0;
@@ -133,11 +131,6 @@ __device__ __forceinline__ cplx kernel_scalar(double dx, double dy, double dz, d
   if constexpr (OP == 0 || OP == 3) {
     return g;
   } else {
-    double dn;
-    if constexpr (OP == 1)
-      dn = dx * ny[0] + dy * ny[1] + dz * ny[2];
-    else
-      dn = -(dx * nx[0] + dy * nx[1] + dz * nx[2]);
     const double f = dn * rinv * rinv * rinv;
     if constexpr (HELM) {
       // e^{ikr}(1 - ikr) = (c + kr s) + i (s - kr c)
@@ -146,6 +139,16 @@ __device__ __forceinline__ cplx kernel_scalar(double dx, double dy, double dz, d
       return {f, 0.0};
     }
   }
+}
+
+// The same for displacement d = x - y; nx: test normal, ny: trial normal.
+template <int OP, bool HELM, int CENTRED = 0>
+__device__ __forceinline__ cplx kernel_scalar(double dx, double dy, double dz, double k, const double* nx,
+                                              const double* ny, double kR = 0.0) {
+  double dn = 0.0;
+  if constexpr (OP == 1) dn = dx * ny[0] + dy * ny[1] + dz * ny[2];
+  if constexpr (OP == 2) dn = -(dx * nx[0] + dy * nx[1] + dz * nx[2]);
+  return kernel_scalar_r2<OP, HELM, CENTRED>(dx * dx + dy * dy + dz * dz, dn, k, kR);
 }
 
 // acc += w G for the single-layer kernel (without 1/(4 pi)); the weight is folded into
@@ -630,18 +633,45 @@ __device__ __forceinline__ cplx p0_pair(const double (&xp)[Q][3], const double (
   return acc;
 }
 
-// Single layer (and the P0 hypersingular n.n term) of one regular P0 cell pair with both
-// rules in orbit form, x_p = c_t + tau_p d'_k'(p), y_q = c_s + t_q d_k(q) (d', d the
+// One regular P0 cell pair (all four operators; the hypersingular one is the single-layer
+// sum, scaled by -k^2 n_x.n_y by the caller) with both rules in orbit form, x_p = c_t + tau_p d'_k'(p), y_q = c_s + t_q d_k(q) (d', d the
 // vertex offsets of the test and trial cell from their centroids, D = c_t - c_s):
 //   r^2 = (R^2 + t_q^2 |d_k|^2) + A_p + t_q W_pk,
 //   A_p = tau_p 2 D.d'_k' + tau_p^2 |d'_k'|^2,  W_pk = -2 D.d_k - 2 tau_p d'_k'.d_k,
 // two instructions per point pair instead of six (three differences and a dot product);
 // W_p2 = -W_p0 - W_p1 and the d'_2 products follow the same way, the offsets of a cell
 // summing to zero. tau2g[p] = tau_p^2 |d'_k'(p)|^2; sv: the trial cell's 12-value record.
-template <int Q, bool HELM, int CENTRED, typename SV>
+template <int Q, int OP, bool HELM, int CENTRED, typename SV>
 __device__ __forceinline__ cplx p0_pair_orbit(const double (&td)[2][3], const double (&tau2g)[Q],
                                               const double (&twc)[3], const SV& sv, const double (&D)[3], double R2,
-                                              double k, double kR) {
+                                              const double (&nx)[3], const double (&ny)[3], double k, double kR) {
+  // double layer: (x - y).n_y = D.n_y + tau_p d'_k'.n_y (o_q lies in the trial plane);
+  // adjoint double layer: -(x - y).n_x = -D.n_x + t_q d_k.n_x (o_p lies in the test plane)
+  double dnp[Q], dnq[Q];
+  if constexpr (OP == 1) {
+    const double Dn = D[0] * ny[0] + D[1] * ny[1] + D[2] * ny[2];
+    double dkn[3];
+    dkn[0] = td[0][0] * ny[0] + td[0][1] * ny[1] + td[0][2] * ny[2];
+    dkn[1] = td[1][0] * ny[0] + td[1][1] * ny[1] + td[1][2] * ny[2];
+    dkn[2] = -(dkn[0] + dkn[1]);
+#pragma unroll
+    for (int p = 0; p < Q; ++p) {
+      const double tau = orbit_t(Q, p);
+      dnp[p] = tau == 0.0 ? Dn : fma(tau, dkn[orbit_vertex(Q, p)], Dn);
+    }
+  }
+  if constexpr (OP == 2) {
+    const double Dn = -(D[0] * nx[0] + D[1] * nx[1] + D[2] * nx[2]);
+    double dkn[3];
+    dkn[0] = sv[0] * nx[0] + sv[1] * nx[1] + sv[2] * nx[2];
+    dkn[1] = sv[4] * nx[0] + sv[5] * nx[1] + sv[6] * nx[2];
+    dkn[2] = -(dkn[0] + dkn[1]);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const double t = orbit_t(Q, q);
+      dnq[q] = t == 0.0 ? Dn : fma(t, dkn[orbit_vertex(Q, q)], Dn);
+    }
+  }
   const double E0 = -2.0 * (D[0] * sv[0] + D[1] * sv[1] + D[2] * sv[2]);
   const double E1 = -2.0 * (D[0] * sv[4] + D[1] * sv[5] + D[2] * sv[6]);
   double N[3][2], F[3];
@@ -677,7 +707,13 @@ __device__ __forceinline__ cplx p0_pair_orbit(const double (&td)[2][3], const do
       const double t = orbit_t(Q, q);
       const double base = tau == 0.0 ? RC[q] : Ap + RC[q];
       const double r2 = t == 0.0 ? base : fma(t, W[orbit_vertex(Q, q)], base);
-      green_accumulate_r2<HELM, CENTRED>(r2, k, kR, sv[8 + weight_class(Q, q)], ap);
+      if constexpr (OP == 0 || OP == 3) {
+        green_accumulate_r2<HELM, CENTRED>(r2, k, kR, sv[8 + weight_class(Q, q)], ap);
+      } else {
+        const cplx kv = kernel_scalar_r2<OP, HELM, CENTRED>(r2, OP == 1 ? dnp[p] : dnq[q], k, kR);
+        ap.re = fma(kv.re, sv[8 + weight_class(Q, q)], ap.re);
+        ap.im = fma(kv.im, sv[8 + weight_class(Q, q)], ap.im);
+      }
     }
     acc.re = fma(ap.re, twc[weight_class(Q, p)], acc.re);
     acc.im = fma(ap.im, twc[weight_class(Q, p)], acc.im);
@@ -697,13 +733,9 @@ __device__ __forceinline__ void p0_tile(Args& A, const int2 tile, int symmetric,
   const int i = tile.x * kTile + lane;
   const bool i_ok = i < A.n_test;
   const int il = i_ok ? i : A.n_test - 1;  // clamped: loads stay in bounds
-  // single layer / hypersingular: rule points in orbit form (p0_pair_orbit); otherwise
-  // the points themselves
-  constexpr bool kOrbit = OP == 0 || OP == 3;
-  constexpr int QX = kOrbit ? 1 : Q;
-  double xp[QX][3], wp[QX];
+  // rule points in orbit form (p0_pair_orbit): the test cell's vertex offsets
   double td[2][3], tau2g[Q], twc[3];
-  if constexpr (kOrbit) {
+  {
     const double* tv = A.test_vtx64 + (size_t)il * 12;
 #pragma unroll
     for (int m = 0; m < 2; ++m)
@@ -715,15 +747,6 @@ __device__ __forceinline__ void p0_tile(Args& A, const int2 tile, int symmetric,
       tau2g[p] = tau * tau * tg[orbit_vertex(Q, p)];
     }
     for (int c = 0; c < 3; ++c) twc[c] = tv[8 + c];
-  } else {
-#pragma unroll
-    for (int p = 0; p < Q; ++p) {
-      const double* s = A.test_pts + ((size_t)il * Q + p) * 4;
-      xp[p][0] = s[0];
-      xp[p][1] = s[1];
-      xp[p][2] = s[2];
-      wp[p] = s[3];
-    }
   }
   const double* tgeo = A.test_geo + (size_t)il * kGeoStride;
   const double nx[3] = {tgeo[9], tgeo[10], tgeo[11]};
@@ -731,7 +754,7 @@ __device__ __forceinline__ void p0_tile(Args& A, const int2 tile, int symmetric,
   const int tv0 = A.test_vid[4 * il], tv1 = A.test_vid[4 * il + 1], tv2 = A.test_vid[4 * il + 2];
   const bool mirror = symmetric && tile.x != tile.y;
   const int skind = fast_store_kind(A.out_type, A.mode);
-  const auto trial_pts = constant_view(A.trial_pts);
+  const auto trial_vtx = constant_view(A.trial_vtx64);
   const auto trial_geo = constant_view(A.trial_geo);
   const auto trial_vid = constant_view(A.trial_vid);
   static_assert(CH == 4 || CH == 8 || CH == 16, "chunk must divide 16 and 64");
@@ -745,11 +768,10 @@ __device__ __forceinline__ void p0_tile(Args& A, const int2 tile, int symmetric,
         continue;
       }
       // wave-uniform trial cell: loaded unconditionally (scalar loads) before any per-lane branch
-      constexpr int NSP = kOrbit ? 12 : Q * 4;
-      const auto spp = kOrbit ? constant_view(A.trial_vtx64) + (size_t)j * 12 : trial_pts + (size_t)j * Q * 4;
-      double sp[NSP];
+      const auto spp = trial_vtx + (size_t)j * 12;
+      double sp[12];
 #pragma unroll
-      for (int m = 0; m < NSP; ++m) sp[m] = spp[m];
+      for (int m = 0; m < 12; ++m) sp[m] = spp[m];
       const auto sg = trial_geo + (size_t)j * kGeoStride;
       const int sv0 = trial_vid[4 * j], sv1 = trial_vid[4 * j + 1], sv2 = trial_vid[4 * j + 2];
       const bool adjacent = A.same_grid && (tv0 == sv0 || tv0 == sv1 || tv0 == sv2 || tv1 == sv0 || tv1 == sv1 ||
@@ -757,24 +779,15 @@ __device__ __forceinline__ void p0_tile(Args& A, const int2 tile, int symmetric,
       const double ny[3] = {sg[9], sg[10], sg[11]};
       cplx acc = {0.0, 0.0};
       if (!adjacent) {
-        if constexpr (kOrbit) {
-          const double cs[3] = {sg[kGeoCentroid], sg[kGeoCentroid + 1], sg[kGeoCentroid + 2]};
-          const double D[3] = {ct[0] - cs[0], ct[1] - cs[1], ct[2] - cs[2]};
-          const double R2 = D[0] * D[0] + D[1] * D[1] + D[2] * D[2];
-          if constexpr (CENTRED) {
-            double kR;
-            const cplx e = pair_phase(ct, cs, A.k, &kR);
-            acc = cmul(p0_pair_orbit<Q, HELM, CENTRED>(td, tau2g, twc, sp, D, R2, A.k, kR), e);
-          } else {
-            acc = p0_pair_orbit<Q, HELM, 0>(td, tau2g, twc, sp, D, R2, A.k, 0.0);
-          }
-        } else if constexpr (CENTRED) {
-          const double cs[3] = {sg[kGeoCentroid], sg[kGeoCentroid + 1], sg[kGeoCentroid + 2]};
+        const double cs[3] = {sg[kGeoCentroid], sg[kGeoCentroid + 1], sg[kGeoCentroid + 2]};
+        const double D[3] = {ct[0] - cs[0], ct[1] - cs[1], ct[2] - cs[2]};
+        const double R2 = D[0] * D[0] + D[1] * D[1] + D[2] * D[2];
+        if constexpr (CENTRED) {
           double kR;
           const cplx e = pair_phase(ct, cs, A.k, &kR);
-          acc = cmul(p0_pair<Q, OP, HELM, CENTRED>(xp, wp, nx, sp, ny, A.k, kR), e);
+          acc = cmul(p0_pair_orbit<Q, OP, HELM, CENTRED>(td, tau2g, twc, sp, D, R2, nx, ny, A.k, kR), e);
         } else {
-          acc = p0_pair<Q, OP, HELM, 0>(xp, wp, nx, sp, ny, A.k, 0.0);
+          acc = p0_pair_orbit<Q, OP, HELM, 0>(td, tau2g, twc, sp, D, R2, nx, ny, A.k, 0.0);
         }
       }
       if constexpr (OP == 3) {  // P0 hypersingular: only the Helmholtz -k^2 n_x.n_y term survives
