@@ -118,6 +118,38 @@ def test_octahedron_matches_reference_shape():
     assert np.abs(g.points - p).max() < 1e-15 and np.array_equal(g.cells, c)
 
 
+def test_rules_in_orbit_form():
+    """The P0 tiles take every rule point as c + t (V_k - c) (c the centroid, V_k a vertex):
+    the centroid (t = 0), or an S21 orbit with repeated barycentric coordinate a, t = 1 - 3a,
+    whose three points put the distinct coordinate on vertex 0, 1, 2 in that order, all with
+    one weight. Checked on the engine's rules against t from the closed-form abscissae (6
+    points: a = (8 - sqrt10 +- sqrt(38 - 44 sqrt(2/5)))/18; 7 points: (6 -+ sqrt15)/21)."""
+    disc = np.sqrt(38.0 - 44.0 * np.sqrt(0.4))
+    a6 = [(8.0 - np.sqrt(10.0) + disc) / 18.0, (8.0 - np.sqrt(10.0) - disc) / 18.0]
+    a7 = [(6.0 - np.sqrt(15.0)) / 21.0, (6.0 + np.sqrt(15.0)) / 21.0]
+    orbits = {1: [None], 3: [1.0 / 6.0], 6: a6, 7: [None] + a7}
+    lib = L.lib()
+    for q, orb in orbits.items():
+        xi = np.zeros(2 * q)
+        w = np.zeros(q)
+        assert lib.bempp_triangle_rule(q, L.ptr(xi), L.ptr(w)) == q
+        lam = np.column_stack([1.0 - xi[0::2] - xi[1::2], xi[0::2], xi[1::2]])
+        p = 0
+        for a in orb:
+            if a is None:  # centroid
+                assert np.abs(lam[p] - 1.0 / 3.0).max() < 1e-15
+                p += 1
+                continue
+            t = 1.0 - 3.0 * a
+            for k in range(3):
+                want = np.full(3, (1.0 - t) / 3.0)
+                want[k] = (1.0 + 2.0 * t) / 3.0
+                assert np.abs(lam[p] - want).max() < 1e-15, (q, p)
+                assert w[p] == w[p - k]
+                p += 1
+        assert p == q
+
+
 def test_rules_match_oracle():
     """The engine's rule tables equal the oracle's (triangle rules and canonical Duffy rules)."""
     lib = L.lib()
