@@ -1,0 +1,45 @@
+"""Synthetic workloads of the benchmark configurations, shared by the benchmark and the tests
+so that both draw identical inputs.
+
+- `batch_spec(seed, n)`: the batched workload, n ellipsoids with icosphere level uniform in
+  {2, 3, 4, 5}, semi-axes uniform in [0.5, 2] and wavenumber uniform in [1, 10]
+  (numpy default_rng(seed), drawn in that order per item).
+- `batch_items(...)`: the same as (grid, DP0 space, assembler, k, level) tuples.
+"""
+import numpy as np
+
+
+def batch_spec(seed, n):
+    """[(level, semi_axes[3], k)] for n items."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        lev = int(rng.integers(2, 6))
+        axes = rng.uniform(0.5, 2.0, 3)
+        k = float(rng.uniform(1.0, 10.0))
+        out.append((lev, axes, k))
+    return out
+
+
+def batch_items(seed, n, dtype=np.complex128, mixed=False, nq=6, nsing=4):
+    """Helmholtz single-layer P0 assemblers on the batch_spec meshes: [(grid, space, assembler,
+    k, level)]. A complex64 assembler holds k in FP32; the k returned is that FP32 value, so a
+    reference assembly with it uses the same wavenumber."""
+    from .assembly.boundary import BoundaryAssemblerOptions, OperatorType, Precision, create_helmholtz_assembler
+    from .element import Continuity, Family, ReferenceCellType, create_family
+    from .function_space import function_space
+    from .shapes import ellipsoid
+
+    items = []
+    for lev, axes, k in batch_spec(seed, n):
+        if np.dtype(dtype) == np.complex64:
+            k = float(np.float32(k))
+        o = BoundaryAssemblerOptions()
+        o.set_regular_quadrature_degree(ReferenceCellType.Triangle, nq)
+        o.set_singular_quadrature_degree(ReferenceCellType.Triangle, ReferenceCellType.Triangle, nsing)
+        if mixed:
+            o.set_precision(Precision.Mixed)
+        g = ellipsoid(lev, axes)
+        s = function_space(g, create_family(Family.Lagrange, 0, Continuity.Discontinuous))
+        items.append((g, s, create_helmholtz_assembler(k, OperatorType.SingleLayer, o, dtype=dtype), k, lev))
+    return items

@@ -63,33 +63,11 @@ def cpu_baseline(points, cells, k, target_s, threads):
 
 
 def batch_items(seed, n, dtype, mixed):
-    """The batched workload: n ellipsoids, level uniform in {2..5}, semi-axes uniform in
-    [0.5, 2], k uniform in [1, 10] (numpy default_rng(seed), drawn in that order per item)."""
-    import numpy as np
+    """The batched workload (bempp_amd.workloads.batch_spec): n ellipsoids, level uniform in
+    {2..5}, semi-axes uniform in [0.5, 2], k uniform in [1, 10]."""
+    from bempp_amd.workloads import batch_items as items
 
-    from bempp_amd.assembly.boundary import (BoundaryAssemblerOptions, OperatorType, Precision,
-                                             create_helmholtz_assembler)
-    from bempp_amd.element import Continuity, Family, ReferenceCellType, create_family
-    from bempp_amd.function_space import function_space
-    from bempp_amd.shapes import ellipsoid
-
-    rng = np.random.default_rng(seed)
-    items = []
-    for _ in range(n):
-        lev = int(rng.integers(2, 6))
-        axes = rng.uniform(0.5, 2.0, 3)
-        k = float(rng.uniform(1.0, 10.0))
-        if dtype == np.complex64:
-            k = float(np.float32(k))  # a complex64 assembler holds k in FP32; the oracle uses the same k
-        o = BoundaryAssemblerOptions()
-        o.set_regular_quadrature_degree(ReferenceCellType.Triangle, 6)
-        o.set_singular_quadrature_degree(ReferenceCellType.Triangle, ReferenceCellType.Triangle, 4)
-        if mixed:
-            o.set_precision(Precision.Mixed)
-        g = ellipsoid(lev, axes)
-        s = function_space(g, create_family(Family.Lagrange, 0, Continuity.Discontinuous))
-        items.append((g, s, create_helmholtz_assembler(k, OperatorType.SingleLayer, o, dtype=dtype), k, lev))
-    return items
+    return items(seed, n, dtype, mixed)
 
 
 def batch_leg(seed, n, steps, mixed):

@@ -513,7 +513,7 @@ struct Job {
   int batch;
   int nthreads;
   bool fast;
-  int64_t row_begin, row_end;  // test-cell range (P0 test space only restricts rows)
+  int64_t row_begin, row_end;  // test-DOF (output row) range; the whole space by default
 };
 
 struct Triplet {
@@ -551,9 +551,18 @@ struct Assembler {
   int64_t ld;   // rows of the output block
   int64_t row0; // first output row (test DOF) held in out
 
-  bool test_in_range(int64_t c) const { return c >= J.row_begin && c < J.row_end; }
+  bool row_in_range(int64_t r) const { return r >= J.row_begin && r < J.row_end; }
+  // a test cell takes part when one of its DOFs is an output row
+  bool test_in_range(int64_t c) const {
+    for (int b = 0; b < test.nb; ++b)
+      if (row_in_range(test.dofs[c * test.nb + b])) return true;
+    return false;
+  }
 
-  void add(int64_t r, int64_t c, T v) { out[(r - row0) + ld * c] += v; }
+  // rows outside the range (other DOFs of a selected P1 cell) are dropped
+  void add(int64_t r, int64_t c, T v) {
+    if (row_in_range(r)) out[(r - row0) + ld * c] += v;
+  }
 
   // src/boundary_assemblers.rs:396-521 and :689-773, cell_pair_assemblers.rs:283-339
   void nonsingular() {
@@ -700,13 +709,10 @@ int assemble_dense_t(const Job& J, const double* pts, int64_t nv, const int64_t*
                      int sdeg, int scont, void* outv) {
   Grid g(pts, nv, cells, nc);
   Space test(&g, tdeg, tcont != 0), trial(&g, sdeg, scont != 0);
-  int64_t ld = test.size, row0 = 0;
-  if (J.row_begin != 0 || J.row_end != nc) {
-    if (tdeg != 0) throw std::runtime_error("row restriction needs a P0 test space");
-    ld = J.row_end - J.row_begin;
-    row0 = J.row_begin;
-  }
-  Assembler<T> A{J, g, trial, test, (T*)outv, ld, row0};
+  Job Jr = J;
+  if (Jr.row_end < 0 || Jr.row_end > test.size) Jr.row_end = test.size;
+  if (Jr.row_begin > Jr.row_end) throw std::runtime_error("row_begin exceeds row_end");
+  Assembler<T> A{Jr, g, trial, test, (T*)outv, Jr.row_end - Jr.row_begin, Jr.row_begin};
   A.nonsingular();
   std::vector<Triplet> idx;
   std::vector<T> vals;
@@ -718,8 +724,8 @@ int assemble_dense_t(const Job& J, const double* pts, int64_t nv, const int64_t*
 Job make_job(int kind, int op, double k, int nq, int nsing, int batch, int nthreads, int fast, int64_t rb, int64_t re,
              int64_t nc) {
   Job J{kind, op, k, nq, nsing, batch, nthreads > 0 ? nthreads : omp_get_max_threads(), fast != 0, rb, re};
+  (void)nc;
   if (J.row_begin < 0) J.row_begin = 0;
-  if (J.row_end < 0 || J.row_end > nc) J.row_end = nc;
   if (kind != 0 && kind != 1) throw std::runtime_error("kind must be 0 (Laplace) or 1 (Helmholtz)");
   if (op < 0 || op > 3) throw std::runtime_error("op must be 0..3");
   if (nsing < 1) throw std::runtime_error("singular quadrature degree must be >= 1");
@@ -734,8 +740,9 @@ extern "C" {
 const char* oracle_last_error() { return g_err.c_str(); }
 
 // Dense assembly, accumulated into `out` (column-major; double for Laplace,
-// complex<double> interleaved for Helmholtz). Rows restricted to test cells in
-// [row_begin,row_end) when the test space is P0 (out then has row_end-row_begin rows).
+// complex<double> interleaved for Helmholtz). Rows restricted to the test DOFs in
+// [row_begin,row_end) (row_end < 0: to the end of the space); out then has
+// row_end-row_begin rows. Only test cells carrying one of those DOFs are visited.
 int oracle_assemble_dense(int kind, int op, double k, const double* pts, int64_t nv, const int64_t* cells, int64_t nc,
                           int test_degree, int test_cont, int trial_degree, int trial_cont, int nq, int nsing, int batch,
                           int nthreads, int fast, int64_t row_begin, int64_t row_end, void* out) {
@@ -755,7 +762,7 @@ int oracle_assemble_singular_coo(int kind, int op, double k, const double* pts, 
                                  int64_t nc, int test_degree, int test_cont, int trial_degree, int trial_cont, int nsing,
                                  int batch, int nthreads, int64_t* count, int64_t* rows, int64_t* cols, void* vals) {
   try {
-    Job J = make_job(kind, op, k, 6, nsing, batch, nthreads, 0, 0, nc, nc);
+    Job J = make_job(kind, op, k, 6, nsing, batch, nthreads, 0, 0, INT64_MAX, nc);
     Grid g(pts, nv, cells, nc);
     Space test(&g, test_degree, test_cont != 0), trial(&g, trial_degree, trial_cont != 0);
     std::vector<Triplet> idx;

@@ -75,14 +75,15 @@ def _space_size(pts, cells, degree, continuous):
 
 def assemble(kind, op, pts, cells, test=(0, False), trial=(0, False), k=0.0, nq=6, nsing=4, batch=128,
              nthreads=0, fast=False, rows=None, out=None):
-    """Dense assembly (accumulated into `out` if given). test/trial = (degree, continuous)."""
+    """Dense assembly (accumulated into `out` if given). test/trial = (degree, continuous).
+    rows = (first, end): only the test-DOF rows first..end-1 (any test space); `out` then has
+    end - first rows."""
     pts = np.ascontiguousarray(pts, dtype=np.float64)
     cells = np.ascontiguousarray(cells, dtype=np.int64)
     nt = _space_size(pts, cells, *test)
     ns = _space_size(pts, cells, *trial)
-    rb, re = (0, len(cells)) if rows is None else rows
-    if rows is not None:
-        nt = re - rb
+    rb, re = (0, nt) if rows is None else (int(rows[0]), min(int(rows[1]), nt))
+    nt = re - rb
     dtype = np.float64 if kind == "laplace" else np.complex128
     if out is None:
         out = np.zeros((nt, ns), dtype=dtype, order="F")

@@ -331,3 +331,54 @@ def test_mixed_precision_shuffled_cells(op):
     print(f"[mixed-shuffled] {op}: shuffled {err:.3e}, sorted {err0:.3e}", flush=True)
     assert 1e-12 < err <= 1e-6 and err0 <= 1e-6
     assert np.abs(A - R).max() / np.abs(R).max() <= 1e-5
+
+
+@pytest.mark.parametrize("nq", [1, 3, 7])
+@pytest.mark.parametrize("kind,k,level", [("laplace", 0.0, 3), ("helmholtz", 1.2, 3), ("helmholtz", 2.5, 3),
+                                          ("helmholtz", 5.0, 2)],
+                         ids=["laplace", "helmholtz_tier2", "helmholtz_tier1", "helmholtz_full_reduction"])
+@pytest.mark.parametrize("op", list(OPS))
+def test_p0_tiles_other_rules(nq, kind, k, level, op):
+    """The FP64 P0 x P0 tiles take both rules' points in orbit form c + t (V_k - c): with one
+    point (the centroid alone, t = 0), three points (one orbit) and seven points (the centroid
+    and two orbits, three weight classes), for all four operators, Laplace and Helmholtz with
+    each phase mode (k = 1.2 and 2.5 on level 3: centred phase tiers 2 and 1; k = 5 on level 2:
+    full reduction per point pair), against the oracle with the same rule."""
+    _check(kind, op, icosphere(level), "dp0", "dp0", k=k, nq=nq)
+
+
+@pytest.mark.parametrize("space", ["dp0", "p1"])
+@pytest.mark.parametrize("kind,op,k", [("laplace", "single_layer", 0.0), ("laplace", "double_layer", 0.0),
+                                       ("helmholtz", "single_layer", 2.0), ("helmholtz", "adjoint_double_layer", 2.0),
+                                       ("helmholtz", "hypersingular", 2.0)])
+def test_fp64_path_single_precision_outputs(kind, op, k, space):
+    """The reference's factories are generic over f32 (Laplace) and c32 (Helmholtz): the FP64
+    path computes in FP64 and stores float32 / complex64. P0 x P0 entries are stored once
+    (the tile's value, or the Duffy value added to the tile's zero), so they equal the FP64
+    matrix rounded to single precision exactly; P1 entries are summed with single-precision
+    atomics (<= 1e-6). assemble_into_memory adds into the caller's single-precision buffer."""
+    g = icosphere(2)
+    s = _space(g, space)
+    o = _options()
+    if kind == "laplace":
+        a32, a64 = create_laplace_assembler(OPS[op], o, dtype=np.float32), create_laplace_assembler(OPS[op], o)
+    else:
+        a32 = create_helmholtz_assembler(k, OPS[op], o, dtype=np.complex64)
+        a64 = create_helmholtz_assembler(k, OPS[op], o)
+    A32 = a32.assemble(s, s)
+    A64 = a64.assemble(s, s)
+    assert A32.dtype == (np.float32 if kind == "laplace" else np.complex64)
+    R = O.assemble(kind, op, g.points, g.cells, test=SPACES[space], trial=SPACES[space], k=k)
+    assert _rel(A64, R) <= TOL
+    err = _rel(A32.astype(A64.dtype), R)
+    print(f"[fp64-f32out] {kind} {op} {space}: rel err vs oracle {err:.3e}", flush=True)
+    assert err <= 1e-6
+    if space == "dp0":
+        assert np.array_equal(A32, A64.astype(A32.dtype))
+    B = np.full_like(A32, 0.5 if kind == "laplace" else 0.5 - 0.25j, order="F")
+    B0 = B.copy()
+    a32.assemble_into_memory(s, s, B)
+    if space == "dp0":
+        assert np.array_equal(B, B0 + A32)  # one single-precision add per entry
+    else:
+        assert np.abs(B - (B0 + A32)).max() <= 1e-6 * np.abs(A32).max() + 1e-6

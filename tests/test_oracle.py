@@ -246,6 +246,23 @@ def test_row_restriction():
     assert np.abs(fast - full).max() / np.abs(full).max() < 1e-14
 
 
+@pytest.mark.parametrize("kind,op,k", [("laplace", "hypersingular", 0.0), ("helmholtz", "double_layer", 5.0),
+                                       ("helmholtz", "adjoint_double_layer", 5.0)])
+@pytest.mark.parametrize("test_sp", [(1, True), (1, False), (0, False)])
+def test_row_restriction_any_test_space(kind, op, k, test_sp):
+    """Row blocks of a P1 (continuous or not) test space: only the test cells carrying one of the
+    selected DOFs are visited and the other DOFs of those cells are dropped; the block equals the
+    same rows of the full matrix (also the last rows, and an end past the space's size)."""
+    p, c = O.sphere(2, "icosahedron")
+    trial = (1, True)
+    full = O.assemble(kind, op, p, c, test=test_sp, trial=trial, k=k)
+    n = full.shape[0]
+    for rb, re in [(0, 7), (40, 73), (n - 9, n + 5)]:
+        part = O.assemble(kind, op, p, c, test=test_sp, trial=trial, k=k, rows=(rb, re))
+        assert part.shape == (min(re, n) - rb, full.shape[1])
+        assert np.abs(part - full[rb:re]).max() <= 1e-15 * np.abs(full).max()
+
+
 @pytest.mark.parametrize("kind,op,k", [("laplace", "single_layer", 0.0), ("helmholtz", "double_layer", 3.0)])
 @pytest.mark.parametrize("deg", [(0, False), (1, True)])
 def test_fmm_identity_on_oracle(kind, op, k, deg):
