@@ -60,6 +60,7 @@ _SIGNATURES = {
     "potential_assemble_into_memory": (c_int, [c_vp, c_vp, c_vp, c_size, c_vp]),
     "potential_assemble_into_device_memory": (c_int, [c_vp, c_vp, c_vp, c_size, c_vp, c_int, c_vp]),
     "boundary_assembler_shape_gradient": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "boundary_assembler_shape_gradient_noalloc": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "matrix_shape0": (c_size, [c_vp]),
     "matrix_shape1": (c_size, [c_vp]),
     "matrix_is_csr": (c_int, [c_vp]),

@@ -555,6 +555,20 @@ int boundary_assembler_shape_gradient(const BoundaryAssemblerT* a, const SpaceT*
       1);
 }
 
+int boundary_assembler_shape_gradient_noalloc(const BoundaryAssemblerT* a, const SpaceT* trial, const SpaceT* test,
+                                              const void* u, const void* v, void* grad, void* stream) {
+  return guarded(
+      [&]() {
+        check_spaces(a, trial, test);
+        need(u, "u");
+        need(v, "v");
+        need(grad, "grad");
+        shape_gradient_noalloc(*a->a, *trial->s, *test->s, u, v, grad, stream);
+        return 0;
+      },
+      1);
+}
+
 int boundary_assembler_apply(const BoundaryAssemblerT* a, const SpaceT* trial, const SpaceT* test, const void* u,
                               void* y, int transpose) {
   return guarded(

@@ -164,6 +164,22 @@ void apply(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& 
 // Shape gradient of v^T A(X) u with respect to the grid's vertex coordinates.
 void shape_gradient(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test, const void* u,
                     const void* v, void* grad);
+// The same with u, v and grad in device memory, enqueued on `stream` (NULL: the null stream),
+// allocating no device memory: its partial results live in a static device buffer of
+// kNoAllocScratchBytes per GPU, so grids of at most kNoAllocMaxCells cells; the grid's,
+// space's and rule's device data must already exist (an ordinary call builds them).
+constexpr long long kNoAllocScratchBytes = 16LL << 20;
+constexpr long long kNoAllocBytesPerCell = 176;  // u, v as complex128 (32 B) + cell-vertex gradient (144 B)
+constexpr long long kNoAllocMaxCells = kNoAllocScratchBytes / kNoAllocBytesPerCell;
+void shape_gradient_noalloc(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test, const void* u,
+                            const void* v, void* grad, void* stream);
+// While alive, the device caches refuse to allocate (an Error instead of an upload).
+struct NoDeviceAllocScope {
+  NoDeviceAllocScope();
+  ~NoDeviceAllocScope();
+  NoDeviceAllocScope(const NoDeviceAllocScope&) = delete;
+  NoDeviceAllocScope& operator=(const NoDeviceAllocScope&) = delete;
+};
 
 int device_count();
 // Elapsed milliseconds of the regular and singular kernels of the last

@@ -103,8 +103,15 @@ int current_device() {
   return d;
 }
 
+// Set while an entry point that must not allocate device memory gathers its inputs:
+// a cache that is not built yet then raises instead of being uploaded.
+thread_local int t_no_device_alloc = 0;
+
 template <typename T>
 T* upload(const std::vector<T>& h) {
+  if (t_no_device_alloc)
+    throw Error("this entry point allocates no device memory, but the device data of this grid, space or rule "
+                "is not built yet: make one ordinary call with the same assembler and spaces first");
   T* d = nullptr;
   HIP_CHECK(hipMalloc((void**)&d, std::max<size_t>(h.size(), 1) * sizeof(T)));
   if (!h.empty()) HIP_CHECK(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
@@ -707,6 +714,9 @@ void launch_singular(int op, bool helm, bool p0p0, bool small, dim3 grid, hipStr
 
 }  // namespace
 
+NoDeviceAllocScope::NoDeviceAllocScope() { ++t_no_device_alloc; }
+NoDeviceAllocScope::~NoDeviceAllocScope() { --t_no_device_alloc; }
+
 void check_assembler(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test) {
   if (a.op != kSL && a.op != kDL && a.op != kADL && a.op != kHS)
     throw Error("operator type not supported for dense assembly");
@@ -716,7 +726,7 @@ void check_assembler(const Assembler& a, const FunctionSpace& trial, const Funct
 }
 
 Prepared prepare(const Assembler& a, const FunctionSpace& trial_c, const FunctionSpace& test_c, void* out,
-                 void* stream) {
+                 void* stream, bool caller_stream) {
   check_assembler(a, trial_c, test_c);
   auto& trial = const_cast<FunctionSpace&>(trial_c);
   auto& test = const_cast<FunctionSpace&>(test_c);
@@ -724,7 +734,7 @@ Prepared prepare(const Assembler& a, const FunctionSpace& trial_c, const Functio
   auto& sgrid = const_cast<Grid&>(*trial.grid);
   Prepared P{};
   P.device = current_device();
-  P.stream = stream ? (hipStream_t)stream : assembler_stream(a, P.device);
+  P.stream = (stream || caller_stream) ? (hipStream_t)stream : assembler_stream(a, P.device);
   P.Q = regular_points(a.options);
   std::vector<double> xi, w;
   triangle_rule(P.Q, xi, w);
@@ -841,8 +851,9 @@ Prepared prepare(const Assembler& a, const FunctionSpace& trial_c, const Functio
 }
 
 // Geometry, rule tables and adjacent pairs for the matrix-free kernels (gradient.hip).
-GradientInputs gradient_inputs(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test) {
-  Prepared P = prepare(a, trial, test, nullptr, nullptr);
+GradientInputs gradient_inputs(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test,
+                               void* stream, bool caller_stream) {
+  Prepared P = prepare(a, trial, test, nullptr, stream, caller_stream);
   GradientInputs g{};
   g.R = P.R;
   g.pairs = P.S.pairs;

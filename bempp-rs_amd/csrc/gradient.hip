@@ -16,6 +16,8 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "core.hpp"
@@ -355,7 +357,8 @@ __global__ void __launch_bounds__(256) grad_singular_kernel(const GradArgs G) {
 // Host side
 // ---------------------------------------------------------------------------
 // Filled by device.hip (shares its caches).
-GradientInputs gradient_inputs(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test);
+GradientInputs gradient_inputs(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test,
+                               void* stream = nullptr, bool caller_stream = false);
 
 namespace {
 
@@ -493,11 +496,54 @@ void apply(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& 
   }
 }
 
-void shape_gradient(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test, const void* u,
-                    const void* v, void* grad) {
+namespace {
+
+void check_shape_gradient(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test) {
   if (a.op != kSL) throw Error("shape gradient is implemented for the single-layer operator");
   if (test.grid != trial.grid) throw Error("shape gradient needs test and trial spaces on one grid");
   if (test.nb != 1 || trial.nb != 1) throw Error("shape gradient is implemented for P0 spaces");
+}
+
+// Enqueues the regular-pair and adjacent-pair gradient kernels on s. G.u, G.v (complex128,
+// nc entries each) and G.gcell (nc x 18 doubles, zeroed) are set by the caller; the
+// geometry, rule and pair fields come from `in`.
+void enqueue_gradient_kernels(const Assembler& a, const GradientInputs& in, int64_t nc, hipStream_t s,
+                              dev::GradArgs& G) {
+  G.R = in.R;
+  G.pairs = in.pairs;
+  G.npairs = in.npairs;
+  G.rule = in.rule;
+  G.rule_total = in.rule_total;
+  for (int mm = 0; mm < 4; ++mm) {
+    G.rule_off[mm] = in.rule_off[mm];
+    G.rule_n[mm] = in.rule_n[mm];
+  }
+  std::vector<double> xi, w;
+  triangle_rule(in.Q, xi, w);
+  for (int p = 0; p < dev::kMaxQ; ++p)
+    for (int k = 0; k < 3; ++k) G.lam[p][k] = 0.0;
+  for (int p = 0; p < in.Q; ++p) {
+    G.lam[p][0] = 1.0 - xi[2 * p] - xi[2 * p + 1];
+    G.lam[p][1] = xi[2 * p];
+    G.lam[p][2] = xi[2 * p + 1];
+  }
+  const int bx = (int)((nc + 255) / 256);
+  const int chunk = (int)std::max<int64_t>(16, std::min<int64_t>(256, (nc + 63) / 64));
+  G.R.trial_chunk = chunk;
+  launch_grad(in.Q, a.kind == kHelmholtz, in.centred, dim3(bx, (unsigned)((nc + chunk - 1) / chunk)), s, G);
+  if (in.npairs > 0) {
+    dim3 sgrid((unsigned)((in.npairs + 3) / 4));
+    if (a.kind == kHelmholtz) dev::grad_singular_kernel<true><<<sgrid, 256, 0, s>>>(G);
+    else dev::grad_singular_kernel<false><<<sgrid, 256, 0, s>>>(G);
+    HIP_CHECK_G(hipGetLastError());
+  }
+}
+
+}  // namespace
+
+void shape_gradient(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test, const void* u,
+                    const void* v, void* grad) {
+  check_shape_gradient(a, trial, test);
   GradientInputs in = gradient_inputs(a, trial, test);
   hipStream_t s = (hipStream_t)in.stream;
   const Grid& g = *test.grid;
@@ -522,37 +568,10 @@ void shape_gradient(const Assembler& a, const FunctionSpace& trial, const Functi
     dev::fill_zero_kernel<<<512, 256, 0, s>>>(dg, (long long)nc * 18);
     HIP_CHECK_G(hipGetLastError());
     dev::GradArgs G{};
-    G.R = in.R;
     G.u = du;
     G.v = dv;
     G.gcell = dg;
-    G.pairs = in.pairs;
-    G.npairs = in.npairs;
-    G.rule = in.rule;
-    G.rule_total = in.rule_total;
-    for (int mm = 0; mm < 4; ++mm) {
-      G.rule_off[mm] = in.rule_off[mm];
-      G.rule_n[mm] = in.rule_n[mm];
-    }
-    std::vector<double> xi, w;
-    triangle_rule(in.Q, xi, w);
-    for (int p = 0; p < dev::kMaxQ; ++p)
-      for (int k = 0; k < 3; ++k) G.lam[p][k] = 0.0;
-    for (int p = 0; p < in.Q; ++p) {
-      G.lam[p][0] = 1.0 - xi[2 * p] - xi[2 * p + 1];
-      G.lam[p][1] = xi[2 * p];
-      G.lam[p][2] = xi[2 * p + 1];
-    }
-    const int bx = (int)((nc + 255) / 256);
-    const int chunk = (int)std::max<int64_t>(16, std::min<int64_t>(256, (nc + 63) / 64));
-    G.R.trial_chunk = chunk;
-    launch_grad(in.Q, a.kind == kHelmholtz, in.centred, dim3(bx, (unsigned)((nc + chunk - 1) / chunk)), s, G);
-    if (in.npairs > 0) {
-      dim3 sgrid((unsigned)((in.npairs + 3) / 4));
-      if (a.kind == kHelmholtz) dev::grad_singular_kernel<true><<<sgrid, 256, 0, s>>>(G);
-      else dev::grad_singular_kernel<false><<<sgrid, 256, 0, s>>>(G);
-      HIP_CHECK_G(hipGetLastError());
-    }
+    enqueue_gradient_kernels(a, in, nc, s, G);
     HIP_CHECK_G(hipMemcpyAsync(hg.data(), dg, gbytes, hipMemcpyDeviceToHost, s));
     HIP_CHECK_G(hipStreamSynchronize(s));
   } catch (...) {
@@ -575,6 +594,119 @@ void shape_gradient(const Assembler& a, const FunctionSpace& trial, const Functi
   } else {
     for (int64_t i = 0; i < nv * 3; ++i) ((double*)grad)[i] = gv[2 * i];
   }
+}
+
+
+// ---------------------------------------------------------------------------
+// Shape gradient that allocates no device memory. Its partial results (u and v
+// as complex128, the per-cell-vertex gradients) live in `scratch`, one static
+// copy per GPU for the life of the process: [u: 2 nc][v: 2 nc][gcell: 18 nc] doubles.
+// Every call on a GPU reuses that GPU's copy and clears what must start at zero;
+// calls that overlap in time on one GPU race on it.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace dev {
+
+// x in the assembler dtype -> interleaved complex128
+__global__ void to_c128_kernel(const void* __restrict__ x, int dtype, long long n, double* __restrict__ out) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    double re, im = 0.0;
+    switch (dtype) {
+      case kF64: re = ((const double*)x)[i]; break;
+      case kF32: re = ((const float*)x)[i]; break;
+      case kC64:
+        re = ((const double*)x)[2 * i];
+        im = ((const double*)x)[2 * i + 1];
+        break;
+      default:
+        re = ((const float*)x)[2 * i];
+        im = ((const float*)x)[2 * i + 1];
+        break;
+    }
+    out[2 * i] = re;
+    out[2 * i + 1] = im;
+  }
+}
+
+// Cell-vertex gradients [nc][3 vertices][3 comps] (complex) -> vertex gradients
+// grad[vertex][comp] (complex128, or float64 real parts), added with atomics into a
+// zeroed output.
+__global__ void scatter_vertex_gradient_kernel(const double* __restrict__ gcell, const int* __restrict__ vid,
+                                               long long nc, double* __restrict__ grad, int complex_out) {
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < 9 * nc;
+       t += (long long)gridDim.x * blockDim.x) {
+    const long long c = t / 9;
+    const int kd = (int)(t - 9 * c);
+    const long long e = (long long)vid[4 * c + kd / 3] * 3 + kd % 3;
+    if (complex_out) {
+      unsafeAtomicAdd(grad + 2 * e, gcell[2 * t]);
+      unsafeAtomicAdd(grad + 2 * e + 1, gcell[2 * t + 1]);
+    } else {
+      unsafeAtomicAdd(grad + e, gcell[2 * t]);
+    }
+  }
+}
+
+}  // namespace dev
+
+namespace {
+
+// Device address of `scratch` on `device` (looked up once per device, with it current).
+double* noalloc_scratch(int device) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static void* ptrs[kMaxDevices] = {};
+  if (device < 0 || device >= kMaxDevices) throw Error("device index out of range");
+  std::lock_guard<std::mutex> lk(mu);
+  if (!ptrs[device]) {
+    void* p = nullptr;
+    HIP_CHECK_G(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw Error("hipGetSymbolAddress returned a null scratch address");
+    ptrs[device] = p;
+  }
+  return (double*)ptrs[device];
+}
+
+}  // namespace
+
+void shape_gradient_noalloc(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test, const void* u,
+                            const void* v, void* grad, void* stream) {
+  check_shape_gradient(a, trial, test);
+  const Grid& g = *test.grid;
+  const int64_t nc = g.nc(), nv = g.nv();
+  if (nc > kNoAllocMaxCells)
+    throw Error("shape gradient without allocation: " + std::to_string(nc) + " cells exceed its static scratch buffer (" +
+                std::to_string(kNoAllocScratchBytes) + " bytes per GPU, at most " + std::to_string(kNoAllocMaxCells) +
+                " cells)");
+  if (device_count() <= 0)
+    throw Error("no HIP device available: assembly requires an MI355X GPU (there is no CPU fallback)");
+  int device = 0;
+  HIP_CHECK_G(hipGetDevice(&device));
+  double* sp = noalloc_scratch(device);
+  GradientInputs in;
+  {
+    NoDeviceAllocScope guard;  // the caches must exist already: nothing is uploaded here
+    in = gradient_inputs(a, trial, test, stream, true);
+  }
+  hipStream_t s = (hipStream_t)stream;
+  double* du = sp;
+  double* dv = sp + 2 * nc;
+  double* dg = sp + 4 * nc;
+  const bool cplx_out = is_complex(a.dtype);
+  // fill kernels, not memset nodes: a caller may capture the stream in a graph
+  dev::to_c128_kernel<<<256, 256, 0, s>>>(u, (int)a.dtype, nc, du);
+  dev::to_c128_kernel<<<256, 256, 0, s>>>(v, (int)a.dtype, nc, dv);
+  dev::fill_zero_kernel<<<512, 256, 0, s>>>(dg, (long long)nc * 18);
+  dev::fill_zero_kernel<<<256, 256, 0, s>>>((double*)grad, (long long)nv * 3 * (cplx_out ? 2 : 1));
+  HIP_CHECK_G(hipGetLastError());
+  dev::GradArgs G{};
+  G.u = du;
+  G.v = dv;
+  G.gcell = dg;
+  enqueue_gradient_kernels(a, in, nc, s, G);
+  dev::scatter_vertex_gradient_kernel<<<512, 256, 0, s>>>(dg, in.R.test_vid, nc, (double*)grad, cplx_out ? 1 : 0);
+  HIP_CHECK_G(hipGetLastError());
 }
 
 }  // namespace bempp_amd

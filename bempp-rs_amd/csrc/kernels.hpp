@@ -1185,7 +1185,9 @@ struct Prepared {
   int npairs;
 };
 
+// stream: the launch stream; NULL selects the assembler's own stream unless caller_stream
+// is set (then NULL is the null stream).
 Prepared prepare(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test, void* out,
-                 void* stream);
+                 void* stream, bool caller_stream = false);
 
 }  // namespace bempp_amd

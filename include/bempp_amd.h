@@ -267,6 +267,21 @@ int boundary_assembler_apply(const BoundaryAssemblerT* assembler, const SpaceT* 
 int boundary_assembler_shape_gradient(const BoundaryAssemblerT* assembler, const SpaceT* trial_space,
                                       const SpaceT* test_space, const void* u, const void* v, void* grad);
 
+/* Shape gradient that allocates no device memory. No counterpart in the reference: it is for
+ * callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). The same result as boundary_assembler_shape_gradient, with u, v
+ * (assembler dtype) and grad (f64 or complex128 [npoints][3], overwritten) in device memory
+ * on the current device; the work is enqueued on `stream` (a hipStream_t; NULL: the null
+ * stream) and the call returns without waiting. Its partial results live in one static
+ * 16 MiB device buffer per GPU (176 B per cell): grids of at most 95325 cells, larger ones
+ * are rejected with an error. Calls on one GPU share that buffer, so calls that overlap in
+ * time on one GPU race: order them on one stream. The grid's, space's and rule's device
+ * data must exist already (any ordinary assembly, apply or shape-gradient call with the same
+ * assembler and spaces builds them); otherwise the call fails instead of allocating. */
+int boundary_assembler_shape_gradient_noalloc(const BoundaryAssemblerT* assembler, const SpaceT* trial_space,
+                                              const SpaceT* test_space, const void* u, const void* v, void* grad,
+                                              void* stream);
+
 /* Kernel timing (no counterpart in the reference): with profiling enabled, each
  * assembly records HIP events around its regular-pair and adjacent-pair kernels
  * on the launch stream; kernel_times waits for them and writes the elapsed

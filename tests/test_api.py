@@ -188,6 +188,22 @@ def test_no_gpu_fails_loudly():
         create_laplace_assembler(OperatorType.SingleLayer, o).assemble(s, s)
 
 
+def test_noalloc_shape_gradient_rejects_oversize_grids():
+    """The no-allocation shape gradient holds 176 B per cell in a 16 MiB static device buffer
+    (at most 95325 cells): a larger grid is refused with an error before any device work
+    (the device pointers are never touched), with or without a GPU."""
+    g = icosphere(7)  # 327680 cells
+    s = function_space(g, create_family(Family.Lagrange, 0, Continuity.Discontinuous))
+    o = BoundaryAssemblerOptions()
+    o.set_regular_quadrature_degree(ReferenceCellType.Triangle, 6)
+    a = create_laplace_assembler(OperatorType.SingleLayer, o)
+    dummy = ctypes.c_void_p(0x1000)
+    rc = L.lib().boundary_assembler_shape_gradient_noalloc(a._rs_assembler, s._rs_space, s._rs_space, dummy, dummy,
+                                                            dummy, None)
+    assert rc != 0
+    assert "exceed its static scratch buffer" in L.last_error() and "95325" in L.last_error()
+
+
 def test_batch_without_gpu_fails_cleanly():
     """The batched entry point reports an error (no crash, no CPU fallback) without a GPU."""
     if L.device_count() > 0:
