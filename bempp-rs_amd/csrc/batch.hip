@@ -121,14 +121,17 @@ void launch_tiles_t(const Batch& b, int blocks, hipStream_t s) {
     const int grid = blocks > 0 ? std::min(nt, blocks) : nt;
     int* ctr = b.counter + g;
     if (b.mixed) {
-      if (!HELM)
+      if constexpr (Q == 0) {
+        throw Error("mixed-precision batches need a rule compiled for its point count");
+      } else if (!HELM) {
         dev::batched_p0_mixed_persistent<Q, OP, HELM, 1, CH><<<grid, 256, 0, s>>>(b.items, tg, nt, ctr, sym);
-      else if (g == 2)
+      } else if (g == 2) {
         dev::batched_p0_mixed_persistent<Q, OP, HELM, 2, CH><<<grid, 256, 0, s>>>(b.items, tg, nt, ctr, sym);
-      else if (g == 1)
+      } else if (g == 1) {
         dev::batched_p0_mixed_persistent<Q, OP, HELM, 1, CH><<<grid, 256, 0, s>>>(b.items, tg, nt, ctr, sym);
-      else
+      } else {
         dev::batched_p0_mixed_persistent<Q, OP, HELM, 0, CH><<<grid, 256, 0, s>>>(b.items, tg, nt, ctr, sym);
+      }
     } else {
       if (!HELM)
         dev::batched_p0_persistent<Q, OP, HELM, 0, CH><<<grid, 256, 0, s>>>(b.items, tg, nt, ctr, sym);
@@ -155,7 +158,7 @@ void launch_tiles_q(const Batch& b, int blocks, hipStream_t s) {
     case 3: launch_tiles_c<3, OP, HELM>(b, blocks, s); break;
     case 6: launch_tiles_c<6, OP, HELM>(b, blocks, s); break;
     case 7: launch_tiles_c<7, OP, HELM>(b, blocks, s); break;
-    default: throw Error("unsupported regular quadrature point count");
+    default: launch_tiles_c<0, OP, HELM>(b, blocks, s); break;  // point count at run time (FP64 only)
   }
 }
 
@@ -258,6 +261,7 @@ Batch* batch_new(const std::vector<const Assembler*>& asms, const std::vector<co
     Prepared P = prepare(a, s, s, outputs[i], nullptr);
     if (P.device != b->device) throw Error("batch items on different devices");
     if (i == 0) b->Q = P.Q;
+    b->mixed = b->mixed && P.mixed;  // mixed precision runs the rules compiled for their point count
     if (P.Q != b->Q) throw Error("batch items must share the regular quadrature degree");
     R[i] = P.R;
     R[i].mode = dev::kStore;

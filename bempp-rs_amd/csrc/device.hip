@@ -534,6 +534,31 @@ const RuleTable& rule_table(int device, int n) {
   return tables.emplace(std::make_pair(device, n), t).first->second;
 }
 
+// Regular rule as a device table [Q][4] = barycentric coordinates (the P1 basis values)
+// and weight of each point, per device and point count; read by the kernels that take
+// the point count at run time.
+const double* regular_rule_table(int device, int Q, const std::vector<double>& xi, const std::vector<double>& w) {
+  static std::mutex mu;
+  static std::map<std::pair<int, int>, double*> tables;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = tables.find({device, Q});
+  if (it != tables.end()) return it->second;
+  std::vector<double> t(4 * (size_t)Q);
+  for (int p = 0; p < Q; ++p) {
+    t[4 * p] = 1.0 - xi[2 * p] - xi[2 * p + 1];
+    t[4 * p + 1] = xi[2 * p];
+    t[4 * p + 2] = xi[2 * p + 1];
+    t[4 * p + 3] = w[p];
+  }
+  double* d = upload(t);
+  tables[{device, Q}] = d;
+  return d;
+}
+
+// Template point count of the kernels for a rule of Q points: Q itself for the rules
+// compiled with their point count (held in registers), 0 (point count at run time) otherwise.
+int kernel_q(int Q) { return (Q == 1 || Q == 3 || Q == 6 || Q == 7) ? Q : 0; }
+
 hipStream_t assembler_stream(const Assembler& a, int device) {
   auto& m = const_cast<Assembler&>(a);
   static std::mutex mu;
@@ -611,12 +636,12 @@ void launch_q(int mode, int Q, dim3 grid, hipStream_t s, const dev::RegularArgs&
   if (mode == 1) dev::regular_p0_tiled<QQ, OP, HELM, CENTRED><<<grid, 256, 0, s>>>(A, tiles, sym);       \
   else if (mode == 2) dev::regular_tiled_lds<QQ, OP, HELM, GC><<<grid, 256, lds, s>>>(A, tiles, sym);    \
   else dev::regular_kernel<QQ, OP, HELM, GC><<<grid, 256, 0, s>>>(A);
-  switch (Q) {
+  switch (kernel_q(Q)) {
     case 1: BEMPP_LAUNCH(1); break;
     case 3: BEMPP_LAUNCH(3); break;
     case 6: BEMPP_LAUNCH(6); break;
     case 7: BEMPP_LAUNCH(7); break;
-    default: throw Error("unsupported regular quadrature point count");
+    default: BEMPP_LAUNCH(0); break;  // point count at run time
   }
 #undef BEMPP_LAUNCH
 }
@@ -768,14 +793,19 @@ Prepared prepare(const Assembler& a, const FunctionSpace& trial_c, const Functio
   dev::RegularArgs& R = P.R;
   R.test_pts = grid_points_for_rule(tgrid, tc, P.Q, xi, w);
   R.trial_pts = grid_points_for_rule(sgrid, sc, P.Q, xi, w);
-  // the FP64 P0 x P0 tiles take both rules' points in orbit form
-  R.orbit_form = rule_in_orbit_form(P.Q, xi, w) ? 1 : 0;
-  if (!R.orbit_form && test.nb == 1 && trial.nb == 1)
+  R.nq = P.Q;
+  R.rule_tab = regular_rule_table(P.device, P.Q, xi, w);
+  // the FP64 P0 x P0 tiles compiled for a point count take both rules' points in orbit
+  // form; other rules run the tiles with the point count at run time
+  const bool compiled_q = kernel_q(P.Q) != 0;
+  R.orbit_form = compiled_q && rule_in_orbit_form(P.Q, xi, w) ? 1 : 0;
+  if (compiled_q && !R.orbit_form && test.nb == 1 && trial.nb == 1)
     throw Error("P0 tiles: quadrature rule not in the tabulated orbit form");
   R.test_vtx64 = R.orbit_form ? grid_vertex64_for_rule(tgrid, tc, P.Q, w) : nullptr;
   R.trial_vtx64 = R.orbit_form ? grid_vertex64_for_rule(sgrid, sc, P.Q, w) : nullptr;
-  // mixed precision: one grid (P0 x P0: p0 mixed tiles; a P1 side: LDS mixed tiles)
-  P.mixed = a.options.precision == kMixed && P.same_grid;
+  // mixed precision: one grid (P0 x P0: p0 mixed tiles; a P1 side: LDS mixed tiles), rules
+  // compiled for their point count; other rules assemble in FP64
+  P.mixed = a.options.precision == kMixed && P.same_grid && compiled_q;
   P.near = nullptr;
   P.nnear = 0;
   P.near_segs = nullptr;
@@ -808,7 +838,7 @@ Prepared prepare(const Assembler& a, const FunctionSpace& trial_c, const Functio
   R.k = a.k;
   for (int p = 0; p < dev::kMaxQ; ++p)
     for (int b = 0; b < 3; ++b) R.phi[p][b] = 0.0;
-  for (int p = 0; p < P.Q; ++p) {
+  for (int p = 0; p < std::min(P.Q, dev::kMaxQ); ++p) {
     R.phi[p][0] = 1.0 - xi[2 * p] - xi[2 * p + 1];
     R.phi[p][1] = xi[2 * p];
     R.phi[p][2] = xi[2 * p + 1];
@@ -1052,7 +1082,7 @@ void launch_correction_q(int Q, dim3 grid, hipStream_t s, const dev::RegularArgs
     case 3: dev::correction_kernel<3, OP, HELM><<<grid, 256, 0, s>>>(R, pairs, n, out); break;
     case 6: dev::correction_kernel<6, OP, HELM><<<grid, 256, 0, s>>>(R, pairs, n, out); break;
     case 7: dev::correction_kernel<7, OP, HELM><<<grid, 256, 0, s>>>(R, pairs, n, out); break;
-    default: throw Error("unsupported regular quadrature point count");
+    default: dev::correction_kernel<0, OP, HELM><<<grid, 256, 0, s>>>(R, pairs, n, out); break;
   }
 }
 
@@ -1124,7 +1154,7 @@ void launch_potential_q(int Q, dim3 grid, hipStream_t s, const dev::PotentialArg
     case 3: dev::potential_kernel<3, OP, HELM><<<grid, 256, 0, s>>>(P); break;
     case 6: dev::potential_kernel<6, OP, HELM><<<grid, 256, 0, s>>>(P); break;
     case 7: dev::potential_kernel<7, OP, HELM><<<grid, 256, 0, s>>>(P); break;
-    default: throw Error("unsupported regular quadrature point count");
+    default: dev::potential_kernel<0, OP, HELM><<<grid, 256, 0, s>>>(P); break;
   }
 }
 
@@ -1154,11 +1184,13 @@ void assemble_potential_device(const Assembler& a, const FunctionSpace& space_c,
   P.k = a.k;
   for (int p = 0; p < dev::kMaxQ; ++p)
     for (int b = 0; b < 3; ++b) P.phi[p][b] = 0.0;
-  for (int p = 0; p < Q; ++p) {
+  for (int p = 0; p < std::min(Q, dev::kMaxQ); ++p) {
     P.phi[p][0] = 1.0 - xi[2 * p] - xi[2 * p + 1];
     P.phi[p][1] = xi[2 * p];
     P.phi[p][2] = xi[2 * p + 1];
   }
+  P.nq = Q;
+  P.rule_tab = regular_rule_table(device, Q, xi, w);
   P.out = dev_out;
   P.out_type = out_type(a.dtype);
   P.mode = space.nb == 1 ? dev::kAccumulate : dev::kAtomic;

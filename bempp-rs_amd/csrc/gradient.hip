@@ -90,21 +90,30 @@ __device__ __forceinline__ void djac_over_j(const double* geo, int k, double out
 }
 
 // Regular pairs: one thread per test cell c, a chunk of trial cells j per block.
+// Q = 0: a rule of any point count (A.nq), test points streamed, barycentric coordinates
+// from A.rule_tab.
 template <int Q, bool HELM, bool CENTRED>
 __global__ void __launch_bounds__(256) grad_regular_kernel(const GradArgs G) {
   const RegularArgs& A = G.R;
+  constexpr bool GEN = Q == 0;
+  constexpr int QR = GEN ? 1 : Q;
+  const int nq = GEN ? A.nq : Q;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   const bool c_ok = c < A.n_test;
   const int cl = c_ok ? c : A.n_test - 1;
-  double xp[Q][3], wp[Q];
+  const double* tpts = A.test_pts + (size_t)cl * nq * 4;
+  double xp[QR][3], wp[QR];
+  if constexpr (!GEN) {
 #pragma unroll
-  for (int p = 0; p < Q; ++p) {
-    const double* s = A.test_pts + ((size_t)cl * Q + p) * 4;
-    xp[p][0] = s[0];
-    xp[p][1] = s[1];
-    xp[p][2] = s[2];
-    wp[p] = s[3];
+    for (int p = 0; p < Q; ++p) {
+      const double* s = tpts + 4 * p;
+      xp[p][0] = s[0];
+      xp[p][1] = s[1];
+      xp[p][2] = s[2];
+      wp[p] = s[3];
+    }
   }
+  const auto tab = constant_view(A.rule_tab);
   const double* tgeo = A.test_geo + (size_t)cl * kGeoStride;
   const double ct[3] = {tgeo[kGeoCentroid], tgeo[kGeoCentroid + 1], tgeo[kGeoCentroid + 2]};
   const int tv0 = A.test_vid[4 * cl], tv1 = A.test_vid[4 * cl + 1], tv2 = A.test_vid[4 * cl + 2];
@@ -123,10 +132,12 @@ __global__ void __launch_bounds__(256) grad_regular_kernel(const GradArgs G) {
   const int j0 = blockIdx.y * A.trial_chunk;
   const int j1 = min(j0 + A.trial_chunk, A.n_trial);
   for (int j = j0; j < j1; ++j) {
-    const auto spp = trial_pts + (size_t)j * Q * 4;
-    double sp[Q * 4];
+    const auto spp = trial_pts + (size_t)j * nq * 4;
+    double sp[QR * 4];
+    if constexpr (!GEN) {
 #pragma unroll
-    for (int m = 0; m < Q * 4; ++m) sp[m] = spp[m];
+      for (int m = 0; m < Q * 4; ++m) sp[m] = spp[m];
+    }
     const auto sg = trial_geo + (size_t)j * kGeoStride;
     const double cs[3] = {sg[kGeoCentroid], sg[kGeoCentroid + 1], sg[kGeoCentroid + 2]};
     const int sv0 = trial_vid[4 * j], sv1 = trial_vid[4 * j + 1], sv2 = trial_vid[4 * j + 2];
@@ -144,15 +155,18 @@ __global__ void __launch_bounds__(256) grad_regular_kernel(const GradArgs G) {
 #pragma unroll
       for (int d = 0; d < 3; ++d) T[k][d] = {0.0, 0.0};
 #pragma unroll
-    for (int p = 0; p < Q; ++p) {
+    for (int p = 0; p < (GEN ? nq : Q); ++p) {
+      const double x0 = GEN ? tpts[4 * p] : xp[p][0], x1 = GEN ? tpts[4 * p + 1] : xp[p][1];
+      const double x2 = GEN ? tpts[4 * p + 2] : xp[p][2], wpp = GEN ? tpts[4 * p + 3] : wp[p];
       cplx Pp = {0.0, 0.0};
       cplx Tp[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
 #pragma unroll
-      for (int q = 0; q < Q; ++q) {
-        const double d[3] = {xp[p][0] - sp[4 * q], xp[p][1] - sp[4 * q + 1], xp[p][2] - sp[4 * q + 2]};
+      for (int q = 0; q < (GEN ? nq : Q); ++q) {
+        const double y0 = GEN ? spp[4 * q] : sp[4 * q], y1 = GEN ? spp[4 * q + 1] : sp[4 * q + 1];
+        const double y2 = GEN ? spp[4 * q + 2] : sp[4 * q + 2], wq = GEN ? spp[4 * q + 3] : sp[4 * q + 3];
+        const double d[3] = {x0 - y0, x1 - y1, x2 - y2};
         cplx g, f;
         green_and_grad<HELM, CENTRED>(d[0], d[1], d[2], A.k, kR, &g, &f);
-        const double wq = sp[4 * q + 3];
         Pp.re = fma(g.re, wq, Pp.re);
         Pp.im = fma(g.im, wq, Pp.im);
         const cplx fw = {f.re * wq, f.im * wq};
@@ -162,11 +176,11 @@ __global__ void __launch_bounds__(256) grad_regular_kernel(const GradArgs G) {
           Tp[dd].im = fma(fw.im, d[dd], Tp[dd].im);
         }
       }
-      P.re = fma(Pp.re, wp[p], P.re);
-      P.im = fma(Pp.im, wp[p], P.im);
+      P.re = fma(Pp.re, wpp, P.re);
+      P.im = fma(Pp.im, wpp, P.im);
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        const double lw = G.lam[p][k] * wp[p];
+        const double lw = (GEN ? tab[4 * p + k] : G.lam[GEN ? 0 : p][k]) * wpp;
 #pragma unroll
         for (int dd = 0; dd < 3; ++dd) {
           T[k][dd].re = fma(lw, Tp[dd].re, T[k][dd].re);
@@ -208,17 +222,23 @@ __global__ void __launch_bounds__(256) grad_regular_kernel(const GradArgs G) {
 template <int Q, int OP, bool HELM, bool CENTRED>
 __global__ void __launch_bounds__(256) apply_regular_kernel(const GradArgs G, double* y) {
   const RegularArgs& A = G.R;
+  constexpr bool GEN = Q == 0;  // a rule of any point count (A.nq), points streamed
+  constexpr int QR = GEN ? 1 : Q;
+  const int nq = GEN ? A.nq : Q;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool i_ok = i < A.n_test;
   const int il = i_ok ? i : A.n_test - 1;
-  double xp[Q][3], wp[Q];
+  const double* tpts = A.test_pts + (size_t)il * nq * 4;
+  double xp[QR][3], wp[QR];
+  if constexpr (!GEN) {
 #pragma unroll
-  for (int p = 0; p < Q; ++p) {
-    const double* s = A.test_pts + ((size_t)il * Q + p) * 4;
-    xp[p][0] = s[0];
-    xp[p][1] = s[1];
-    xp[p][2] = s[2];
-    wp[p] = s[3];
+    for (int p = 0; p < Q; ++p) {
+      const double* s = tpts + 4 * p;
+      xp[p][0] = s[0];
+      xp[p][1] = s[1];
+      xp[p][2] = s[2];
+      wp[p] = s[3];
+    }
   }
   const double* tgeo = A.test_geo + (size_t)il * kGeoStride;
   const double nx[3] = {tgeo[9], tgeo[10], tgeo[11]};
@@ -232,10 +252,12 @@ __global__ void __launch_bounds__(256) apply_regular_kernel(const GradArgs G, do
   const int j0 = blockIdx.y * A.trial_chunk;
   const int j1 = min(j0 + A.trial_chunk, A.n_trial);
   for (int j = j0; j < j1; ++j) {
-    const auto spp = trial_pts + (size_t)j * Q * 4;
-    double sp[Q * 4];
+    const auto spp = trial_pts + (size_t)j * nq * 4;
+    double sp[QR * 4];
+    if constexpr (!GEN) {
 #pragma unroll
-    for (int m = 0; m < Q * 4; ++m) sp[m] = spp[m];
+      for (int m = 0; m < Q * 4; ++m) sp[m] = spp[m];
+    }
     const auto sgp = trial_geo + (size_t)j * kGeoStride;
     double sg[kGeoStride];
 #pragma unroll
@@ -247,13 +269,14 @@ __global__ void __launch_bounds__(256) apply_regular_kernel(const GradArgs G, do
     if (adjacent || !i_ok) continue;
     const double ny[3] = {sg[9], sg[10], sg[11]};
     cplx v;
-    if constexpr (CENTRED) {
-      double kR;
-      const cplx e = pair_phase(ct, sg + kGeoCentroid, A.k, &kR);
-      v = cmul(p0_pair<Q, OP, HELM, CENTRED>(xp, wp, nx, sp, ny, A.k, kR), e);
-    } else {
-      v = p0_pair<Q, OP, HELM, 0>(xp, wp, nx, sp, ny, A.k, 0.0);
-    }
+    double kR = 0.0;
+    cplx e = {1.0, 0.0};
+    if constexpr (CENTRED) e = pair_phase(ct, sg + kGeoCentroid, A.k, &kR);
+    if constexpr (GEN)
+      v = p0_pair_generic<OP, HELM, CENTRED>(tpts, nq, nx, spp, ny, A.k, kR);
+    else
+      v = p0_pair<Q, OP, HELM, CENTRED>(xp, wp, nx, sp, ny, A.k, kR);
+    if constexpr (CENTRED) v = cmul(v, e);
     if constexpr (OP == 3) {
       const double nn = HELM ? -A.k * A.k * (nx[0] * ny[0] + nx[1] * ny[1] + nx[2] * ny[2]) : 0.0;
       v = {nn * v.re, nn * v.im};
@@ -377,7 +400,7 @@ void launch_grad(int Q, bool helm, bool centred, dim3 grid, hipStream_t s, const
     case 3: BEMPP_G(3); break;
     case 6: BEMPP_G(6); break;
     case 7: BEMPP_G(7); break;
-    default: throw Error("unsupported regular quadrature point count");
+    default: BEMPP_G(0); break;  // any other rule: point count at run time
   }
 #undef BEMPP_G
   HIP_CHECK_G(hipGetLastError());
@@ -422,7 +445,7 @@ void launch_apply_op(int op, int Q, dim3 grid, hipStream_t s, const dev::GradArg
     case 3: BEMPP_A(3); break;
     case 6: BEMPP_A(6); break;
     case 7: BEMPP_A(7); break;
-    default: throw Error("unsupported regular quadrature point count");
+    default: BEMPP_A(0); break;  // any other rule: point count at run time
   }
 #undef BEMPP_A
   HIP_CHECK_G(hipGetLastError());
@@ -522,7 +545,7 @@ void enqueue_gradient_kernels(const Assembler& a, const GradientInputs& in, int6
   triangle_rule(in.Q, xi, w);
   for (int p = 0; p < dev::kMaxQ; ++p)
     for (int k = 0; k < 3; ++k) G.lam[p][k] = 0.0;
-  for (int p = 0; p < in.Q; ++p) {
+  for (int p = 0; p < std::min(in.Q, dev::kMaxQ); ++p) {
     G.lam[p][0] = 1.0 - xi[2 * p] - xi[2 * p + 1];
     G.lam[p][1] = xi[2 * p];
     G.lam[p][2] = xi[2 * p + 1];

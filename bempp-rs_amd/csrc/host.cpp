@@ -171,12 +171,70 @@ void orbit21(std::vector<double>& xi, std::vector<double>& w, double a, double w
   xi.insert(xi.end(), pts, pts + 6);
   w.insert(w.end(), 3, wt);
 }
+
+// orbit of the barycentric point (a, b, 1 - a - b) with three distinct coordinates: its
+// six permutations (l0, l1, l2), as reference coordinates (xi0, xi1) = (l1, l2)
+void orbit111(std::vector<double>& xi, std::vector<double>& w, double a, double b, double wt) {
+  const double l[3] = {a, b, 1.0 - a - b};
+  static const int perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+  for (const auto& p : perm) {
+    xi.push_back(l[p[1]]);
+    xi.push_back(l[p[2]]);
+  }
+  w.insert(w.end(), 6, wt);
+}
+
+// Orbit tables {type, a, b, weight}: type 0 the centroid, 1 an S21 orbit (a, a, 1 - 2a),
+// 2 an S111 orbit (a, b, 1 - a - b). Derived from the moment equations by
+// triangle_rules.py in this directory (positive weights, interior points; the
+// 37-point rule is the degree-13 solution of smallest truncation error).
+struct RuleOrbit {
+  int type;
+  double a, b, w;
+};
+// 16 points, degree 8: centroid, 3 S21, 1 S111 (Dunavant's degree-8 rule)
+const RuleOrbit kRule16[] = {
+    {0, 0.333333333333333333333, 0.0, 0.0721578038388935841255},
+    {1, 0.0505472283170309754584, 0.0, 0.0162292488115990401555},
+    {1, 0.170569307751760206622, 0.0, 0.0516086852673591251409},
+    {1, 0.459292588292723156029, 0.0, 0.0475458171336423123969},
+    {2, 0.00839477740995760533721, 0.263112829634638113422, 0.0136151570872174971324},
+};
+// 37 points, degree 13: centroid, 4 S21, 4 S111
+const RuleOrbit kRule37[] = {
+    {0, 0.333333333333333333333, 0.0, 0.0333326559198216055272},
+    {1, 0.124457822063999260968, 0.0, 0.0162728885505310468142},
+    {1, 0.225840072867902017418, 0.0, 0.0285184397656679532653},
+    {1, 0.429159612543110726445, 0.0, 0.028185691589537655882},
+    {1, 0.487428645297366234185, 0.0, 0.013523851440530054372},
+    {2, 0.00493532348954305527152, 0.286214753544342015383, 0.00456921940718551612603},
+    {2, 0.0163507805075914482666, 0.0328542486808598094305, 0.00197048267071738041324},
+    {2, 0.0267328097943362938005, 0.124525415851328238663, 0.00875670102545966496751},
+    {2, 0.0712744715119110429253, 0.284520764019818232792, 0.0192310519035338157386},
+};
+
+template <size_t N>
+void expand_orbits(const RuleOrbit (&t)[N], std::vector<double>& xi, std::vector<double>& w) {
+  for (const RuleOrbit& o : t) {
+    if (o.type == 0) {
+      xi.push_back(1.0 / 3.0);
+      xi.push_back(1.0 / 3.0);
+      w.push_back(o.w);
+    } else if (o.type == 1) {
+      orbit21(xi, w, o.a, o.w);
+    } else {
+      orbit111(xi, w, o.a, o.b, o.w);
+    }
+  }
+}
 }  // namespace
 
-// Symmetric triangle rules (the Witherden-Vincent rules of these point counts):
+// Symmetric triangle rules (ndelement's rules are called at src/boundary_assemblers.rs:419-464):
 // 1 point degree 1; 3 points degree 2; 6 points degree 4 (two S21 orbits with the
 // closed-form abscissae a = (8 - sqrt10 +- sqrt(38 - 44 sqrt(2/5)))/18); 7 points
-// degree 5 (Radon: centroid + orbits a = (6 -+ sqrt15)/21).
+// degree 5 (Radon: centroid + orbits a = (6 -+ sqrt15)/21) -- these four coincide with the
+// Witherden-Vincent rules of the same point counts; 16 points degree 8 and 37 points
+// degree 13 (the reference's default) from the orbit tables above.
 void triangle_rule(int npoints, std::vector<double>& xi, std::vector<double>& w) {
   xi.clear();
   w.clear();
@@ -197,9 +255,13 @@ void triangle_rule(int npoints, std::vector<double>& xi, std::vector<double>& w)
     w = {9.0 / 80.0};
     orbit21(xi, w, (6.0 - root15) / 21.0, (155.0 - root15) / 2400.0);
     orbit21(xi, w, (6.0 + root15) / 21.0, (155.0 + root15) / 2400.0);
+  } else if (npoints == 16) {
+    expand_orbits(kRule16, xi, w);
+  } else if (npoints == 37) {
+    expand_orbits(kRule37, xi, w);
   } else {
     throw Error("unsupported regular quadrature point count " + std::to_string(npoints) +
-                " for triangles (supported: 1, 3, 6, 7)");
+                " for triangles (supported: 1, 3, 6, 7, 16, 37)");
   }
 }
 

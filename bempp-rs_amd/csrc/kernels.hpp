@@ -28,6 +28,10 @@ constexpr double kInv4Pi = 0.079577471545947667884441881686257181;  // 1/(4 pi)
 constexpr int kGeoStride = 32;
 constexpr int kGeoCentroid = 24;
 constexpr int kGeoRadius = 27;
+// Regular rules of up to kMaxQ points are compiled with the point count as a template
+// parameter (points held in registers); any other rule runs the same kernels instantiated
+// with Q = 0, which take the point count at run time (RegularArgs::nq) and stream the
+// points from memory.
 constexpr int kMaxQ = 7;
 constexpr int kTile = 64;   // cells per tile side (lanes of a wavefront)
 constexpr int kChunk = 8;   // trial cells per chunk of the P0 tile kernel
@@ -68,6 +72,10 @@ struct RegularArgs {
   int trial_chunk;
   double k;
   double phi[kMaxQ][3];     // basis values at the regular points
+  // the regular rule, any point count: [nq][4] = P1 basis values phi_0..phi_2 (the
+  // barycentric coordinates) and the weight at each point
+  const double* rule_tab;
+  int nq;
   // rule weight x basis value per side (a P0 side: {w, 0, 0}), FP32 (mixed-precision P1 tiles)
   float wphi_test[kMaxQ][3];
   float wphi_trial[kMaxQ][3];
@@ -406,6 +414,62 @@ __device__ __forceinline__ void p1_local_streamed(Args& A, const double* tpts, c
     }
 }
 
+// p1_local for a rule of any point count (A.nq): test points streamed from tpts, trial
+// points and basis values (A.rule_tab) read with wave-uniform loads point by point.
+template <int OP, bool HELM, int CENTRED, typename Args, typename SP, typename SG>
+__device__ __forceinline__ void p1_local_generic(Args& A, const double* tpts, const double (&nx)[3],
+                                                 const double (&ct)[3], SP spp, SG sgeo, cplx (&M)[3][3],
+                                                 cplx& msum) {
+  const double ny[3] = {sgeo[9], sgeo[10], sgeo[11]};
+  double kR = 0.0;
+  cplx e = {1.0, 0.0};
+  if constexpr (CENTRED) {
+    const double cs[3] = {sgeo[kGeoCentroid], sgeo[kGeoCentroid + 1], sgeo[kGeoCentroid + 2]};
+    e = pair_phase(ct, cs, A.k, &kR);
+  }
+  const auto tab = constant_view(A.rule_tab);
+  const int nq = A.nq;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) M[a][b] = {0.0, 0.0};
+#pragma unroll 1
+  for (int p = 0; p < nq; ++p) {
+    const double x0 = tpts[4 * p], x1 = tpts[4 * p + 1], x2 = tpts[4 * p + 2], w = tpts[4 * p + 3];
+    cplx T[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll 2
+    for (int q = 0; q < nq; ++q) {
+      const cplx kv = kernel_scalar<OP, HELM, CENTRED>(x0 - spp[4 * q], x1 - spp[4 * q + 1], x2 - spp[4 * q + 2],
+                                                       A.k, nx, ny, kR);
+      const double wq = spp[4 * q + 3];
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        const double f = (A.nbs == 3 ? tab[4 * q + b] : (b == 0 ? 1.0 : 0.0)) * wq;
+        T[b].re = fma(kv.re, f, T[b].re);
+        T[b].im = fma(kv.im, f, T[b].im);
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double f = (A.nbt == 3 ? tab[4 * p + a] : (a == 0 ? 1.0 : 0.0)) * w;
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        M[a][b].re = fma(f, T[b].re, M[a][b].re);
+        M[a][b].im = fma(f, T[b].im, M[a][b].im);
+      }
+    }
+  }
+  msum = {0.0, 0.0};
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      if constexpr (CENTRED) M[a][b] = cmul(M[a][b], e);
+      msum.re += M[a][b].re;
+      msum.im += M[a][b].im;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Regular pairs with a P1 space on either side. One thread per test cell; the
 // block's trial cells are walked in order with wave-uniform (scalar) loads of
@@ -413,18 +477,23 @@ __device__ __forceinline__ void p1_local_streamed(Args& A, const double* tpts, c
 // ---------------------------------------------------------------------------
 template <int Q, int OP, bool HELM, bool CENTRED>
 __global__ void __launch_bounds__(256) regular_kernel(const RegularArgs A) {
+  constexpr bool GEN = Q == 0;       // rule of any point count, points streamed
+  constexpr int QR = GEN ? 1 : Q;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool i_ok = i < A.n_test;
   const int il = i_ok ? i : A.n_test - 1;  // clamped: loads stay in bounds
-  double xp[Q][3], wp[Q];
+  double xp[QR][3], wp[QR];
+  if constexpr (!GEN) {
 #pragma unroll
-  for (int p = 0; p < Q; ++p) {
-    const double* s = A.test_pts + ((size_t)il * Q + p) * 4;
-    xp[p][0] = s[0];
-    xp[p][1] = s[1];
-    xp[p][2] = s[2];
-    wp[p] = s[3];
+    for (int p = 0; p < Q; ++p) {
+      const double* s = A.test_pts + ((size_t)il * Q + p) * 4;
+      xp[p][0] = s[0];
+      xp[p][1] = s[1];
+      xp[p][2] = s[2];
+      wp[p] = s[3];
+    }
   }
+  const double* tpts = A.test_pts + (size_t)il * (GEN ? A.nq : Q) * 4;
   const double* tgeo = A.test_geo + (size_t)il * kGeoStride;
   const double nx[3] = {tgeo[9], tgeo[10], tgeo[11]};
   const double ct[3] = {tgeo[kGeoCentroid], tgeo[kGeoCentroid + 1], tgeo[kGeoCentroid + 2]};
@@ -435,17 +504,22 @@ __global__ void __launch_bounds__(256) regular_kernel(const RegularArgs A) {
   const int j0 = blockIdx.y * A.trial_chunk;
   const int j1 = min(j0 + A.trial_chunk, A.n_trial);
   for (int j = j0; j < j1; ++j) {
-    const auto spp = trial_pts + (size_t)j * Q * 4;
-    double sp[Q * 4];
+    const auto spp = trial_pts + (size_t)j * (GEN ? A.nq : Q) * 4;
+    double sp[QR * 4];
+    if constexpr (!GEN) {
 #pragma unroll
-    for (int m = 0; m < Q * 4; ++m) sp[m] = spp[m];
+      for (int m = 0; m < Q * 4; ++m) sp[m] = spp[m];
+    }
     const auto sg = trial_geo + (size_t)j * kGeoStride;
     const int sv0 = trial_vid[4 * j], sv1 = trial_vid[4 * j + 1], sv2 = trial_vid[4 * j + 2];
     const bool adjacent = A.same_grid && (tv0 == sv0 || tv0 == sv1 || tv0 == sv2 || tv1 == sv0 || tv1 == sv1 ||
                                           tv1 == sv2 || tv2 == sv0 || tv2 == sv1 || tv2 == sv2);
     if (adjacent || !i_ok) continue;
     cplx M[3][3], msum;
-    p1_local<Q, OP, HELM, CENTRED>(A, xp, wp, nx, ct, sp, sg, M, msum);
+    if constexpr (GEN)
+      p1_local_generic<OP, HELM, CENTRED>(A, tpts, nx, ct, spp, sg, M, msum);
+    else
+      p1_local<Q, OP, HELM, CENTRED>(A, xp, wp, nx, ct, sp, sg, M, msum);
     for (int b = 0; b < A.nbs; ++b) {
       const long long col = A.trial_dofs[(size_t)j * A.nbs + b];
       for (int a = 0; a < A.nbt; ++a) {
@@ -510,9 +584,11 @@ __global__ void __launch_bounds__(256) regular_tiled_lds(const RegularArgs A, co
   const int il = i_ok ? i : A.n_test - 1;
   // Helmholtz: test points read per point (fewer registers, more waves); Laplace: held in
   // registers (its kernel is short enough that the register version runs at higher occupancy)
-  const double* tpts = A.test_pts + (size_t)il * Q * 4;
-  double xp[Q][3], wp[Q];
-  if constexpr (!HELM) {
+  constexpr bool GEN = Q == 0;  // rule of any point count, points streamed
+  constexpr int QR = GEN ? 1 : Q;
+  const double* tpts = A.test_pts + (size_t)il * (GEN ? A.nq : Q) * 4;
+  double xp[QR][3], wp[QR];
+  if constexpr (!HELM && !GEN) {
 #pragma unroll
     for (int p = 0; p < Q; ++p) {
       xp[p][0] = tpts[4 * p];
@@ -537,10 +613,12 @@ __global__ void __launch_bounds__(256) regular_tiled_lds(const RegularArgs A, co
   for (int jj = 0; jj < 16; ++jj) {
     const int j = tile.y * kTile + wave * 16 + jj;
     if (j >= A.n_trial) break;  // wave-uniform
-    const auto spp = trial_pts + (size_t)j * Q * 4;
-    double sp[Q * 4];
+    const auto spp = trial_pts + (size_t)j * (GEN ? A.nq : Q) * 4;
+    double sp[QR * 4];
+    if constexpr (!GEN) {
 #pragma unroll
-    for (int m = 0; m < Q * 4; ++m) sp[m] = spp[m];
+      for (int m = 0; m < Q * 4; ++m) sp[m] = spp[m];
+    }
     const auto sg = trial_geo + (size_t)j * kGeoStride;
     const int sv0 = trial_vid[4 * j], sv1 = trial_vid[4 * j + 1], sv2 = trial_vid[4 * j + 2];
     int ls[3] = {0, 0, 0};
@@ -551,7 +629,9 @@ __global__ void __launch_bounds__(256) regular_tiled_lds(const RegularArgs A, co
                                           tv1 == sv2 || tv2 == sv0 || tv2 == sv1 || tv2 == sv2);
     if (adjacent || !i_ok) continue;
     cplx M[3][3], msum;
-    if constexpr (HELM)
+    if constexpr (GEN)
+      p1_local_generic<OP, HELM, CENTRED>(A, tpts, nx, ct, spp, sg, M, msum);
+    else if constexpr (HELM)
       p1_local_streamed<Q, OP, HELM, CENTRED>(A, tpts, nx, ct, sp, sg, M, msum);
     else
       p1_local<Q, OP, HELM, CENTRED>(A, xp, wp, nx, ct, sp, sg, M, msum);
@@ -721,6 +801,35 @@ __device__ __forceinline__ cplx p0_pair_orbit(const double (&td)[2][3], const do
   return acc;
 }
 
+// One regular P0 cell pair for a rule of any point count (nq): the lane's test points
+// streamed from tpts (x, y, z, w J per point), the wave-uniform trial cell's points read
+// with scalar loads point by point.
+template <int OP, bool HELM, int CENTRED, typename SP>
+__device__ __forceinline__ cplx p0_pair_generic(const double* tpts, int nq, const double (&nx)[3], SP spp,
+                                                const double (&ny)[3], double k, double kR) {
+  cplx acc = {0.0, 0.0};
+#pragma unroll 1
+  for (int p = 0; p < nq; ++p) {
+    const double x0 = tpts[4 * p], x1 = tpts[4 * p + 1], x2 = tpts[4 * p + 2], w = tpts[4 * p + 3];
+    cplx ap = {0.0, 0.0};
+#pragma unroll 4
+    for (int q = 0; q < nq; ++q) {
+      if constexpr (OP == 0 || OP == 3) {
+        green_accumulate<HELM, CENTRED>(x0 - spp[4 * q], x1 - spp[4 * q + 1], x2 - spp[4 * q + 2], k, kR,
+                                        spp[4 * q + 3], ap);
+      } else {
+        const cplx kv = kernel_scalar<OP, HELM, CENTRED>(x0 - spp[4 * q], x1 - spp[4 * q + 1], x2 - spp[4 * q + 2],
+                                                         k, nx, ny, kR);
+        ap.re = fma(kv.re, spp[4 * q + 3], ap.re);
+        ap.im = fma(kv.im, spp[4 * q + 3], ap.im);
+      }
+    }
+    acc.re = fma(ap.re, w, acc.re);
+    acc.im = fma(ap.im, w, acc.im);
+  }
+  return acc;
+}
+
 // CH: trial cells per chunk (a wavefront covers 16 trial cells in 16 / CH chunks);
 // the mirror transpose stores CH contiguous entries per segment.
 // Args: RegularArgs, possibly qualified with the constant address space (batched
@@ -733,9 +842,13 @@ __device__ __forceinline__ void p0_tile(Args& A, const int2 tile, int symmetric,
   const int i = tile.x * kTile + lane;
   const bool i_ok = i < A.n_test;
   const int il = i_ok ? i : A.n_test - 1;  // clamped: loads stay in bounds
+  // Q = 0: a rule of any point count (A.nq), points streamed (p0_pair_generic)
+  constexpr bool GEN = Q == 0;
+  constexpr int QR = GEN ? 1 : Q;
+  const double* tpts = A.test_pts + (size_t)il * A.nq * 4;
   // rule points in orbit form (p0_pair_orbit): the test cell's vertex offsets
-  double td[2][3], tau2g[Q], twc[3];
-  {
+  double td[2][3], tau2g[QR], twc[3];
+  if constexpr (!GEN) {
     const double* tv = A.test_vtx64 + (size_t)il * 12;
 #pragma unroll
     for (int m = 0; m < 2; ++m)
@@ -755,6 +868,7 @@ __device__ __forceinline__ void p0_tile(Args& A, const int2 tile, int symmetric,
   const bool mirror = symmetric && tile.x != tile.y;
   const int skind = fast_store_kind(A.out_type, A.mode);
   const auto trial_vtx = constant_view(A.trial_vtx64);
+  const auto trial_pts = constant_view(A.trial_pts);
   const auto trial_geo = constant_view(A.trial_geo);
   const auto trial_vid = constant_view(A.trial_vid);
   static_assert(CH == 4 || CH == 8 || CH == 16, "chunk must divide 16 and 64");
@@ -768,10 +882,13 @@ __device__ __forceinline__ void p0_tile(Args& A, const int2 tile, int symmetric,
         continue;
       }
       // wave-uniform trial cell: loaded unconditionally (scalar loads) before any per-lane branch
-      const auto spp = trial_vtx + (size_t)j * 12;
       double sp[12];
+      if constexpr (!GEN) {
+        const auto spp = trial_vtx + (size_t)j * 12;
 #pragma unroll
-      for (int m = 0; m < 12; ++m) sp[m] = spp[m];
+        for (int m = 0; m < 12; ++m) sp[m] = spp[m];
+      }
+      const auto gpp = trial_pts + (size_t)j * A.nq * 4;
       const auto sg = trial_geo + (size_t)j * kGeoStride;
       const int sv0 = trial_vid[4 * j], sv1 = trial_vid[4 * j + 1], sv2 = trial_vid[4 * j + 2];
       const bool adjacent = A.same_grid && (tv0 == sv0 || tv0 == sv1 || tv0 == sv2 || tv1 == sv0 || tv1 == sv1 ||
@@ -782,7 +899,15 @@ __device__ __forceinline__ void p0_tile(Args& A, const int2 tile, int symmetric,
         const double cs[3] = {sg[kGeoCentroid], sg[kGeoCentroid + 1], sg[kGeoCentroid + 2]};
         const double D[3] = {ct[0] - cs[0], ct[1] - cs[1], ct[2] - cs[2]};
         const double R2 = D[0] * D[0] + D[1] * D[1] + D[2] * D[2];
-        if constexpr (CENTRED) {
+        if constexpr (GEN) {
+          if constexpr (CENTRED) {
+            double kR;
+            const cplx e = pair_phase(ct, cs, A.k, &kR);
+            acc = cmul(p0_pair_generic<OP, HELM, CENTRED>(tpts, A.nq, nx, gpp, ny, A.k, kR), e);
+          } else {
+            acc = p0_pair_generic<OP, HELM, 0>(tpts, A.nq, nx, gpp, ny, A.k, 0.0);
+          }
+        } else if constexpr (CENTRED) {
           double kR;
           const cplx e = pair_phase(ct, cs, A.k, &kR);
           acc = cmul(p0_pair_orbit<Q, OP, HELM, CENTRED>(td, tau2g, twc, sp, D, R2, nx, ny, A.k, kR), e);
@@ -1040,20 +1165,23 @@ __global__ void __launch_bounds__(256) correction_kernel(const RegularArgs A, co
   for (int a = 0; a < 3; ++a)
 #pragma unroll
     for (int b = 0; b < 3; ++b) M[a][b] = {0.0, 0.0};
-  for (int p = 0; p < Q; ++p) {
-    const double* xs = A.test_pts + ((size_t)c * Q + p) * 4;
-    for (int q = 0; q < Q; ++q) {
-      const double* ys = A.trial_pts + ((size_t)d * Q + q) * 4;
+  // Q = 0: a rule of any point count (A.nq), basis values from A.rule_tab
+  const int nq = Q > 0 ? Q : A.nq;
+  auto phi = [&](int p, int a) { return Q > 0 ? A.phi[p][a] : A.rule_tab[4 * p + a]; };
+  for (int p = 0; p < nq; ++p) {
+    const double* xs = A.test_pts + ((size_t)c * nq + p) * 4;
+    for (int q = 0; q < nq; ++q) {
+      const double* ys = A.trial_pts + ((size_t)d * nq + q) * 4;
       const double dx = xs[0] - ys[0], dy = xs[1] - ys[1], dz = xs[2] - ys[2];
       if (dx == 0.0 && dy == 0.0 && dz == 0.0) continue;
       const cplx kv = kernel_scalar<OP, HELM, 0>(dx, dy, dz, A.k, nx, ny);
       const double w = xs[3] * ys[3];
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        const double fa = (A.nbt == 3 ? A.phi[p][a] : (a == 0 ? 1.0 : 0.0)) * w;
+        const double fa = (A.nbt == 3 ? phi(p, a) : (a == 0 ? 1.0 : 0.0)) * w;
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
-          const double f = fa * (A.nbs == 3 ? A.phi[q][b] : (b == 0 ? 1.0 : 0.0));
+          const double f = fa * (A.nbs == 3 ? phi(q, b) : (b == 0 ? 1.0 : 0.0));
           M[a][b].re = fma(kv.re, f, M[a][b].re);
           M[a][b].im = fma(kv.im, f, M[a][b].im);
         }
@@ -1095,6 +1223,8 @@ struct PotentialArgs {
   int n_trial, nbs, trial_chunk;
   double k;
   double phi[kMaxQ][3];
+  const double* rule_tab;   // [nq][4] basis values and weights (rules of any point count)
+  int nq;
   void* out;                // [np x ndofs] column-major
   int out_type;
   int mode;                 // kAccumulate (P0: one thread per entry) or kAtomic (P1)
@@ -1111,18 +1241,22 @@ __global__ void __launch_bounds__(256) potential_kernel(const PotentialArgs A) {
   const auto trial_dofs = constant_view(A.trial_dofs);
   const int j0 = blockIdx.y * A.trial_chunk;
   const int j1 = min(j0 + A.trial_chunk, A.n_trial);
+  // Q = 0: a rule of any point count (A.nq), basis values from A.rule_tab
+  const int nq = Q > 0 ? Q : A.nq;
+  const auto tab = constant_view(A.rule_tab);
   for (int j = j0; j < j1; ++j) {
-    const auto sp = trial_pts + (size_t)j * Q * 4;
+    const auto sp = trial_pts + (size_t)j * nq * 4;
     const auto sg = trial_geo + (size_t)j * kGeoStride;
     const double ny[3] = {sg[9], sg[10], sg[11]};
     cplx T[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
+    for (int q = 0; q < (Q > 0 ? Q : nq); ++q) {
       const cplx kv = kernel_scalar<OP, HELM, 0>(x[0] - sp[4 * q], x[1] - sp[4 * q + 1], x[2] - sp[4 * q + 2], A.k,
                                                  ny, ny);
 #pragma unroll
       for (int b = 0; b < 3; ++b) {
-        const double f = (A.nbs == 3 ? A.phi[q][b] : (b == 0 ? 1.0 : 0.0)) * sp[4 * q + 3];
+        const double ph = Q > 0 ? A.phi[q < kMaxQ ? q : 0][b] : tab[4 * q + b];
+        const double f = (A.nbs == 3 ? ph : (b == 0 ? 1.0 : 0.0)) * sp[4 * q + 3];
         T[b].re = fma(kv.re, f, T[b].re);
         T[b].im = fma(kv.im, f, T[b].im);
       }

@@ -215,9 +215,52 @@ void add_s21(std::vector<double>& p, std::vector<double>& w, double a, double wt
   for (auto& q : xy) { p.push_back(q[0]); p.push_back(q[1]); w.push_back(wt); }
 }
 
+// (a, b, 1 - a - b) with distinct coordinates: all six barycentric permutations
+void add_s111(std::vector<double>& p, std::vector<double>& w, double a, double b, double wt) {
+  const double l[3] = {a, b, 1.0 - a - b};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      if (j != i) {
+        p.push_back(l[j]);           // xi0 = barycentric coordinate of vertex 1
+        p.push_back(l[3 - i - j]);   // xi1 = barycentric coordinate of vertex 2
+        w.push_back(wt);
+      }
+}
+
+// 16-point degree-8 and 37-point degree-13 fully symmetric rules: orbit parameters from the
+// solution of their moment equations (positive weights, interior points; for 37 points the
+// solution of smallest degree-14 truncation error). ndelement's own digits are not
+// available: parity with them is unpinned; exactness is pinned by the monomial tests.
+void table_rule(const double (*t)[4], int n, std::vector<double>& p, std::vector<double>& w) {
+  for (int i = 0; i < n; ++i) {
+    const int type = (int)t[i][0];
+    if (type == 0) { p.push_back(1.0 / 3.0); p.push_back(1.0 / 3.0); w.push_back(t[i][3]); }
+    else if (type == 1) add_s21(p, w, t[i][1], t[i][3]);
+    else add_s111(p, w, t[i][1], t[i][2], t[i][3]);
+  }
+}
+
 void regular_rule(int npts, std::vector<double>& p, std::vector<double>& w) {
   p.clear(); w.clear();
+  static const double r16[][4] = {
+      {0, 0, 0, 0.0721578038388935841255},
+      {1, 0.0505472283170309754584, 0, 0.0162292488115990401555},
+      {1, 0.170569307751760206622, 0, 0.0516086852673591251409},
+      {1, 0.459292588292723156029, 0, 0.0475458171336423123969},
+      {2, 0.00839477740995760533721, 0.263112829634638113422, 0.0136151570872174971324}};
+  static const double r37[][4] = {
+      {0, 0, 0, 0.0333326559198216055272},
+      {1, 0.124457822063999260968, 0, 0.0162728885505310468142},
+      {1, 0.225840072867902017418, 0, 0.0285184397656679532653},
+      {1, 0.429159612543110726445, 0, 0.028185691589537655882},
+      {1, 0.487428645297366234185, 0, 0.013523851440530054372},
+      {2, 0.00493532348954305527152, 0.286214753544342015383, 0.00456921940718551612603},
+      {2, 0.0163507805075914482666, 0.0328542486808598094305, 0.00197048267071738041324},
+      {2, 0.0267328097943362938005, 0.124525415851328238663, 0.00875670102545966496751},
+      {2, 0.0712744715119110429253, 0.284520764019818232792, 0.0192310519035338157386}};
   switch (npts) {
+    case 16: table_rule(r16, 5, p, w); return;
+    case 37: table_rule(r37, 9, p, w); return;
     case 1: p = {1.0 / 3.0, 1.0 / 3.0}; w = {0.5}; break;
     case 3: add_s21(p, w, 1.0 / 6.0, 1.0 / 6.0); break;
     case 6: {

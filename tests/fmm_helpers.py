@@ -56,9 +56,9 @@ def adjacency_mask(cells, Q):
     return np.kron(adjacent, np.ones((Q, Q), dtype=bool))
 
 
-def regular_parts(kind, op, k, points, cells, dofs, ndofs):
+def regular_parts(kind, op, k, points, cells, dofs, ndofs, npts=6):
     """(P^T K P, regular-rule sums of the adjacent pairs only)."""
-    x, wj, n, xi = quadrature(points, cells)
+    x, wj, n, xi = quadrature(points, cells, npts)
     Q = x.shape[1]
     P = basis_to_quadrature(dofs, ndofs, xi, wj)
     K = kernel_matrix(kind, op, k, x.reshape(-1, 3), np.repeat(n, Q, axis=0))

@@ -44,20 +44,30 @@ def test_golden_python_single_layer():
     assert np.allclose(A, _golden(case), rtol=1e-4)
 
 
-@pytest.mark.parametrize("npts,degree", [(1, 1), (3, 2), (6, 4), (7, 5)])
+@pytest.mark.parametrize("npts,degree", [(1, 1), (3, 2), (6, 4), (7, 5), (16, 8), (37, 13)])
 def test_regular_rule_exact(npts, degree):
-    """Symmetric rules integrate x^a y^b exactly up to their degree: a!b!/(a+b+2)!."""
+    """Symmetric rules integrate x^a y^b exactly up to their degree: a!b!/(a+b+2)!; positive
+    weights, interior points, invariant under the six permutations of the barycentric
+    coordinates. 16 and 37 points: the degree-8 and degree-13 rules derived from their moment
+    equations (bempp-rs_amd/csrc/triangle_rules.py); the reference's default is 37 points."""
     p, w = O.regular_rule(npts)
     assert len(w) == npts and abs(w.sum() - 0.5) < 1e-15
-    assert np.all(p > 0) and np.all(p.sum(1) < 1)
+    assert np.all(w > 0) and np.all(p > 0) and np.all(p.sum(1) < 1)
+    lam = np.column_stack([1 - p.sum(1), p[:, 0], p[:, 1]])
+    for perm in [(1, 0, 2), (0, 2, 1), (2, 1, 0)]:  # the point set maps onto itself
+        q = lam[:, perm]
+        d = np.abs(q[:, None, :] - lam[None, :, :]).max(-1)
+        match = d.argmin(1)
+        assert d.min(1).max() < 1e-15 and np.array_equal(np.sort(match), np.arange(npts))
+        assert np.abs(w[match] - w).max() < 1e-17
     for a in range(degree + 1):
         for b in range(degree + 1 - a):
             exact = factorial(a) * factorial(b) / factorial(a + b + 2)
-            assert abs(np.sum(w * p[:, 0] ** a * p[:, 1] ** b) - exact) < 1e-15
+            assert abs(np.sum(w * p[:, 0] ** a * p[:, 1] ** b) - exact) < 2e-16
     # the next degree is not integrated exactly by all monomials (degree is sharp)
-    errs = [abs(np.sum(w * p[:, 0] ** a * p[:, 1] ** (degree + 1 - a)) -
-                factorial(a) * factorial(degree + 1 - a) / factorial(degree + 3)) for a in range(degree + 2)]
-    assert max(errs) > 1e-6
+    errs = [abs(np.sum(w * p[:, 0] ** a * p[:, 1] ** (degree + 1 - a)) * factorial(degree + 3) /
+                (factorial(a) * factorial(degree + 1 - a)) - 1.0) for a in range(degree + 2)]
+    assert max(errs) > 1e-5
 
 
 def test_regular_rule_unsupported():
