@@ -1,9 +1,12 @@
 """ctypes binding of libbempp_amd.so (the C ABI declared in include/bempp_amd.h)."""
 import ctypes
+import os
 from pathlib import Path
 
 _HERE = Path(__file__).resolve().parent
-_LIB_FILE = _HERE / "libbempp_amd.so"
+# BEMPP_AMD_LIBRARY selects another build of the same library (e.g. the sanitizer build
+# of the host code, make -C bempp-rs_amd/csrc sanitize)
+_LIB_FILE = Path(os.environ["BEMPP_AMD_LIBRARY"]) if os.environ.get("BEMPP_AMD_LIBRARY") else _HERE / "libbempp_amd.so"
 
 c_size = ctypes.c_size_t
 c_u8 = ctypes.c_uint8

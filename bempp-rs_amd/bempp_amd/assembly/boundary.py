@@ -268,7 +268,7 @@ def create_laplace_assembler(operator_type: OperatorType, options: typing.Option
 
 def create_helmholtz_assembler(wavenumber: float, operator_type: OperatorType,
                                options: typing.Optional[BoundaryAssemblerOptions] = None, dtype=np.complex128):
-    """Create a Helmholtz assembler (Rust: helmholtz::assembler::*(k, &options), src/helmholtz.rs:122-186)."""
+    """Create a Helmholtz assembler (Rust: helmholtz::assembler::*(k, &options), src/helmholtz.rs:44-107)."""
     options = BoundaryAssemblerOptions() if options is None else options
     rt = _rtype(dtype)
     k = np.array([wavenumber], dtype=np.float32 if rt == DTYPE_C32 else np.float64)

@@ -4,7 +4,7 @@
 //   BoundaryAssemblerOptions      src/boundary_assemblers.rs:32-100
 //   BoundaryAssembler             src/boundary_assemblers.rs:102-231
 //   FunctionSpace (serial part)   src/function.rs:240-445
-//   laplace/helmholtz factories   src/laplace.rs:34-78, src/helmholtz.rs:122-186
+//   laplace/helmholtz factories   src/laplace.rs:34-78, src/helmholtz.rs:44-107
 //   regular_sphere                src/shapes.rs:19-114
 #pragma once
 

@@ -37,11 +37,12 @@ struct DeviceGridCache {
   std::map<int, float*> pts32;    // regular point count -> [nc][Q][4] FP32 offsets from the centroid, w*jdet
   std::map<int, float*> vtx32;    // regular point count -> [nc][3][4] FP32 vertex offsets, class weights
   std::map<int, double*> vtx64;   // the same in FP64
-  int2* near = nullptr;           // non-adjacent pairs closer than the mixed-precision far bound (superset),
-  int nnear = -1;                 // sorted by (test block, trial block) of 64 cells
+  int2* near = nullptr;           // non-adjacent pairs the mixed-precision far test rejects,
+  int nnear = -1;                 // grouped by (test block, trial block) of 64 cells
   int4* near_segs = nullptr;      // per block pair with near pairs: test block, trial block, first, count
   int nsegs = 0;
   unsigned char* far_tiles = nullptr;  // [nblocks][nblocks]: every cell pair of the tile is far
+  std::vector<unsigned char> far_tiles_host;
   int far_tiles_ld = 0;
 };
 
@@ -106,6 +107,17 @@ int current_device() {
 // Set while an entry point that must not allocate device memory gathers its inputs:
 // a cache that is not built yet then raises instead of being uploaded.
 thread_local int t_no_device_alloc = 0;
+
+// Uninitialised device array of n elements (at least one).
+template <typename T>
+T* device_array(size_t n) {
+  if (t_no_device_alloc)
+    throw Error("this entry point allocates no device memory, but the device data of this grid, space or rule "
+                "is not built yet: make one ordinary call with the same assembler and spaces first");
+  T* d = nullptr;
+  HIP_CHECK(hipMalloc((void**)&d, std::max<size_t>(n, 1) * sizeof(T)));
+  return d;
+}
 
 template <typename T>
 T* upload(const std::vector<T>& h) {
@@ -304,89 +316,6 @@ bool rule_in_orbit_form(int Q, const std::vector<double>& xi, const std::vector<
   return true;
 }
 
-// Non-adjacent cell pairs with centroid distance below 1.01 x the far bound
-// kNearFactor (rad_i + rad_j), found with a uniform bucket grid over centroids.
-// A superset of the pairs the mixed tile kernel leaves out; the near-pair kernel
-// re-applies the exact device predicate.
-std::vector<int2> near_pairs_host(const Grid& g) {
-  const int64_t nc = g.nc();
-  std::vector<double> cen(3 * nc), rad(nc);
-  double rmax = 0.0, lo[3] = {1e300, 1e300, 1e300};
-  for (int64_t i = 0; i < nc; ++i) {
-    double r2 = 0.0;
-    for (int d = 0; d < 3; ++d)
-      cen[3 * i + d] = (g.points[3 * g.cells[3 * i] + d] + g.points[3 * g.cells[3 * i + 1] + d] +
-                        g.points[3 * g.cells[3 * i + 2] + d]) / 3.0;
-    for (int m = 0; m < 3; ++m) {
-      double s = 0.0;
-      for (int d = 0; d < 3; ++d) {
-        const double t = g.points[3 * g.cells[3 * i + m] + d] - cen[3 * i + d];
-        s += t * t;
-      }
-      r2 = std::max(r2, s);
-    }
-    rad[i] = std::sqrt(r2) * (1.0 + 1e-12);
-    rmax = std::max(rmax, rad[i]);
-    for (int d = 0; d < 3; ++d) lo[d] = std::min(lo[d], cen[3 * i + d]);
-  }
-  const double factor = dev::kNearFactor * 1.01;
-  const double h = std::max(factor * 2.0 * rmax, 1e-300);
-  std::map<std::tuple<int64_t, int64_t, int64_t>, std::vector<int64_t>> buckets;
-  auto key = [&](int64_t i) {
-    return std::make_tuple((int64_t)std::floor((cen[3 * i] - lo[0]) / h), (int64_t)std::floor((cen[3 * i + 1] - lo[1]) / h),
-                           (int64_t)std::floor((cen[3 * i + 2] - lo[2]) / h));
-  };
-  for (int64_t i = 0; i < nc; ++i) buckets[key(i)].push_back(i);
-  std::vector<int2> out;
-  for (int64_t i = 0; i < nc; ++i) {
-    const auto k = key(i);
-    const int64_t* tv = &g.cells[3 * i];
-    for (int dx = -1; dx <= 1; ++dx)
-      for (int dy = -1; dy <= 1; ++dy)
-        for (int dz = -1; dz <= 1; ++dz) {
-          auto it = buckets.find(std::make_tuple(std::get<0>(k) + dx, std::get<1>(k) + dy, std::get<2>(k) + dz));
-          if (it == buckets.end()) continue;
-          for (int64_t j : it->second) {
-            const int64_t* sv = &g.cells[3 * j];
-            bool adj = false;
-            for (int a = 0; a < 3 && !adj; ++a)
-              for (int b = 0; b < 3; ++b)
-                if (tv[a] == sv[b]) { adj = true; break; }
-            if (adj) continue;
-            double R2 = 0.0;
-            for (int d = 0; d < 3; ++d) R2 += (cen[3 * i + d] - cen[3 * j + d]) * (cen[3 * i + d] - cen[3 * j + d]);
-            const double lim = factor * (rad[i] + rad[j]);
-            if (R2 < lim * lim) out.push_back(make_int2((int)i, (int)j));
-          }
-        }
-  }
-  return out;
-}
-
-const int2* grid_near_pairs(Grid& g, DeviceGridCache* c, int* n, const int4** segs, int* nsegs) {
-  std::lock_guard<std::mutex> lk(g.cache_mutex);
-  if (c->nnear < 0) {
-    std::vector<int2> v = near_pairs_host(g);
-    // group by (test block, trial block) of kTile cells: the P1 near kernel sums each group in LDS
-    auto key = [](const int2& p) { return std::make_tuple(p.x / dev::kTile, p.y / dev::kTile, p.x, p.y); };
-    std::sort(v.begin(), v.end(), [&](const int2& a, const int2& b) { return key(a) < key(b); });
-    std::vector<int4> sg;
-    for (size_t i = 0; i < v.size(); ++i) {
-      const int tx = v[i].x / dev::kTile, ty = v[i].y / dev::kTile;
-      if (sg.empty() || sg.back().x != tx || sg.back().y != ty) sg.push_back(make_int4(tx, ty, (int)i, 0));
-      ++sg.back().w;
-    }
-    c->near = upload(v);
-    c->near_segs = upload(sg);
-    c->nnear = (int)v.size();
-    c->nsegs = (int)sg.size();
-  }
-  *n = c->nnear;
-  if (segs) *segs = c->near_segs;
-  if (nsegs) *nsegs = c->nsegs;
-  return c->near;
-}
-
 // Tiles (64-cell blocks a, b) whose cell pairs are all far for the mixed-precision split:
 // with block centre C (mean centroid), rho (largest centroid distance from C) and rmax
 // (largest cell radius), every pair has R >= |C_a - C_b| - rho_a - rho_b; the tile is
@@ -427,10 +356,83 @@ const unsigned char* grid_far_tiles(Grid& g, DeviceGridCache* c, int* ld) {
         flags[(size_t)a * nb + b] = lower >= dev::kNearFactor * 1.001 * (rmax[a] + rmax[b]) ? 1 : 0;
       }
     c->far_tiles = upload(flags);
+    c->far_tiles_host = std::move(flags);
     c->far_tiles_ld = nb;
   }
   *ld = c->far_tiles_ld;
   return c->far_tiles;
+}
+
+// The mixed-precision near-pair list of a grid (non-adjacent pairs the far test rejects),
+// built on the device from the candidate tiles (those not flagged all-far; see
+// near_count_kernel / near_fill_kernel), grouped by tile: segs[s] = {test block, trial
+// block, first pair, pair count} for each tile with near pairs.
+const int2* grid_near_pairs(Grid& g, DeviceGridCache* c, int* n, const int4** segs, int* nsegs) {
+  int ld = 0;
+  (void)grid_far_tiles(g, c, &ld);
+  std::lock_guard<std::mutex> lk(g.cache_mutex);
+  if (c->nnear < 0) {
+    const int nb = c->far_tiles_ld;
+    std::vector<int2> cand;
+    for (int a = 0; a < nb; ++a)
+      for (int b = 0; b < nb; ++b)
+        if (!c->far_tiles_host[(size_t)a * nb + b]) cand.push_back(make_int2(a, b));
+    std::vector<int> counts(cand.size()), offsets(cand.size());
+    std::vector<int4> sg;
+    long long total = 0;
+    int2* tiles = nullptr;
+    int* dcounts = nullptr;
+    int* doffsets = nullptr;
+    int2* out = nullptr;
+    auto cleanup = [&]() {
+      (void)hipFree(tiles);
+      (void)hipFree(dcounts);
+      (void)hipFree(doffsets);
+    };
+    try {
+      if (!cand.empty()) {
+        tiles = upload(cand);
+        dcounts = upload(counts);
+        dev::NearBuildArgs B{};
+        B.geo = c->geo;
+        B.vid = c->vid;
+        B.n = (int)g.nc();
+        B.tiles = tiles;
+        B.counts = dcounts;
+        dev::near_count_kernel<><<<(unsigned)cand.size(), 256, 0, nullptr>>>(B);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipMemcpy(counts.data(), dcounts, counts.size() * sizeof(int), hipMemcpyDeviceToHost));
+        for (size_t t = 0; t < cand.size(); ++t) {
+          offsets[t] = (int)total;
+          if (counts[t] > 0) sg.push_back(make_int4(cand[t].x, cand[t].y, (int)total, counts[t]));
+          total += counts[t];
+          if (total > INT32_MAX) throw Error("too many near pairs for the mixed-precision split");
+        }
+        if (total > 0) {
+          doffsets = upload(offsets);
+          out = device_array<int2>((size_t)total);
+          B.offsets = doffsets;
+          B.out = out;
+          dev::near_fill_kernel<><<<(unsigned)cand.size(), 256, 0, nullptr>>>(B);
+          HIP_CHECK(hipGetLastError());
+          HIP_CHECK(hipStreamSynchronize(nullptr));
+        }
+      }
+      c->near_segs = upload(sg);
+    } catch (...) {
+      cleanup();
+      (void)hipFree(out);
+      throw;
+    }
+    cleanup();
+    c->near = out ? out : device_array<int2>(1);
+    c->nnear = (int)total;
+    c->nsegs = (int)sg.size();
+  }
+  *n = c->nnear;
+  if (segs) *segs = c->near_segs;
+  if (nsegs) *nsegs = c->nsegs;
+  return c->near;
 }
 
 const int4* grid_pairs(Grid& g, DeviceGridCache* c, int* npairs) {

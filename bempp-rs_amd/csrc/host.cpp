@@ -53,13 +53,13 @@ void Grid::set_points(const double* pts) {
 }
 
 // Sphere by repeated midpoint refinement with projection to radius 1
-// (src/shapes.rs:246-288): children [c0,e2,e1], [c1,e0,e2], [c2,e1,e0],
+// (src/shapes.rs:59-101): children [c0,e2,e1], [c1,e0,e2], [c2,e1,e0],
 // [e0,e1,e2] with e0=mid(c1,c2), e1=mid(c0,c2), e2=mid(c0,c1); midpoints are
 // numbered on first creation.
 Grid* make_sphere(int seed, uint32_t level) {
   std::vector<double> p;
   std::vector<int64_t> c;
-  if (seed == 0) {  // octahedron, src/shapes.rs:225-243
+  if (seed == 0) {  // octahedron, src/shapes.rs:38-56
     p = {0, 0, 1, 1, 0, 0, 0, 1, 0, -1, 0, 0, 0, -1, 0, 0, 0, -1};
     c = {0, 1, 2, 0, 2, 3, 0, 3, 4, 0, 4, 1, 5, 2, 1, 5, 3, 2, 5, 4, 3, 5, 1, 4};
   } else if (seed == 1) {  // icosahedron with outward-oriented faces

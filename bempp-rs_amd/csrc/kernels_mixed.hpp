@@ -597,6 +597,77 @@ __global__ void __launch_bounds__(256) regular_tiled_lds_mixed(const RegularArgs
   p1_tile_mixed<Q, OP, HELM, PH>(A, tiles[blockIdx.x], symmetric, acc);
 }
 
+// ---------------------------------------------------------------------------
+// The near-pair list of one grid, built on the device: for every tile (blocks a, b of
+// kTile cells) not flagged all-far, its non-adjacent cell pairs that far_pair rejects --
+// the same predicate, evaluated the same way, as the mixed tiles use, so the list holds
+// exactly the pairs the tiles leave out. Two passes over the candidate tiles (count, then
+// fill at the offsets the host computed from the counts); thread t of a tile's 256 takes
+// test cell kTile a + t % 64 and trial cells kTile b + 16 (t / 64) + 0..15, and writes its
+// pairs in that order after the pairs of threads < t (deterministic order).
+// ---------------------------------------------------------------------------
+struct NearBuildArgs {
+  const double* geo;   // [n][kGeoStride]
+  const int* vid;      // [n][4]
+  int n;
+  const int2* tiles;   // candidate tiles (a, b)
+  int* counts;         // [ntiles] pairs per tile (count pass)
+  const int* offsets;  // [ntiles] first pair of each tile (fill pass)
+  int2* out;
+};
+
+__device__ __forceinline__ int near_build_thread(const NearBuildArgs& B, int2 tile, int t, int2* out) {
+  const int i = tile.x * kTile + (t & 63);
+  if (i >= B.n) return 0;
+  const double* tg = B.geo + (size_t)i * kGeoStride;
+  const double ct[3] = {tg[kGeoCentroid], tg[kGeoCentroid + 1], tg[kGeoCentroid + 2]};
+  const double trad = tg[kGeoRadius];
+  const int tv0 = B.vid[4 * i], tv1 = B.vid[4 * i + 1], tv2 = B.vid[4 * i + 2];
+  int cnt = 0;
+  for (int jj = 0; jj < 16; ++jj) {
+    const int j = tile.y * kTile + (t >> 6) * 16 + jj;
+    if (j >= B.n) break;
+    const double* sg = B.geo + (size_t)j * kGeoStride;
+    const int sv0 = B.vid[4 * j], sv1 = B.vid[4 * j + 1], sv2 = B.vid[4 * j + 2];
+    const bool adjacent = tv0 == sv0 || tv0 == sv1 || tv0 == sv2 || tv1 == sv0 || tv1 == sv1 || tv1 == sv2 ||
+                          tv2 == sv0 || tv2 == sv1 || tv2 == sv2;
+    double R2;
+    if (!adjacent && !far_pair(ct, sg + kGeoCentroid, trad, sg[kGeoRadius], &R2)) {
+      if (out) out[cnt] = make_int2(i, j);
+      ++cnt;
+    }
+  }
+  return cnt;
+}
+
+// (templates so that every translation unit including this header may instantiate them)
+template <typename Args = NearBuildArgs>
+__global__ void __launch_bounds__(256) near_count_kernel(const Args B) {
+  __shared__ int total;
+  if (threadIdx.x == 0) total = 0;
+  __syncthreads();
+  const int c = near_build_thread(B, B.tiles[blockIdx.x], threadIdx.x, nullptr);
+  if (c) atomicAdd(&total, c);
+  __syncthreads();
+  if (threadIdx.x == 0) B.counts[blockIdx.x] = total;
+}
+
+template <typename Args = NearBuildArgs>
+__global__ void __launch_bounds__(256) near_fill_kernel(const Args B) {
+  __shared__ int scan[256];
+  const int2 tile = B.tiles[blockIdx.x];
+  const int c = near_build_thread(B, tile, threadIdx.x, nullptr);
+  scan[threadIdx.x] = c;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {  // inclusive prefix sum over the block
+    const int v = threadIdx.x >= (unsigned)off ? scan[threadIdx.x - off] : 0;
+    __syncthreads();
+    scan[threadIdx.x] += v;
+    __syncthreads();
+  }
+  if (c) near_build_thread(B, tile, threadIdx.x, B.out + B.offsets[blockIdx.x] + scan[threadIdx.x] - c);
+}
+
 // FP64 evaluation of the near (non-adjacent) cell pairs with a P1 side, grouped by
 // (test block, trial block) of kTile cells: one workgroup per group sums its pairs' local
 // blocks in LDS at block-local DOF indices (as regular_tiled_lds does) and adds the touched
@@ -655,7 +726,7 @@ __device__ __forceinline__ void near_pair(Args& A, const int2 pr) {
   const double* tg = A.test_geo + (size_t)i * kGeoStride;
   const double* sg = A.trial_geo + (size_t)j * kGeoStride;
   double R2;
-  // the host list is a superset; keep exactly the pairs the tile kernel left out
+  // re-check with the tiles' predicate (the list holds exactly the pairs they leave out)
   if (far_pair(tg + kGeoCentroid, sg + kGeoCentroid, tg[kGeoRadius], sg[kGeoRadius], &R2)) return;
   double xp[Q][3], wp[Q], sp[Q * 4];
 #pragma unroll

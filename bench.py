@@ -22,11 +22,25 @@ HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E
 # 1/r: 1, kr: 1, sin, cos: 2, e^{ikr}/r: 2, weight: 1, complex accumulate: 4) and per pair epilogue
 FLOP_PER_QPP = 20
 FLOP_PER_PAIR_EPILOGUE = 4
-# HBM bytes per launch of the FP64 tile kernel at the default workload (level 5, k = 5),
-# from rocprofv3 --pmc FETCH_SIZE (x 2.00) and WRITE_SIZE (x 1.00), the factors being what
-# the counters read for a 1 GiB device copy (bench.py --copy-calibration); see
-# profiles/regular_p0_tiled_helmholtz_sl_p0_l5_k5.md
-TILE_KERNEL_TRAFFIC_L5_K5 = 1.092e9 + 6.711e9
+# rocprofv3 summary of the FP64 tile kernel at the default workload (level 5, k = 5): its
+# "traffic_bytes_per_launch:" line is FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 of one dispatch,
+# the factors being what the counters read for a 1 GiB device copy (--copy-calibration),
+# measured with the same build; it is rewritten whenever the kernel changes.
+TILE_PROFILE = ROOT / "profiles" / "regular_p0_tiled_helmholtz_sl_p0_l5_k5.md"
+
+
+def profiled_traffic():
+    """(bytes per launch, source) from the committed counter summary, or (None, reason)."""
+    import re
+
+    try:
+        m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", TILE_PROFILE.read_text(), flags=re.M)
+    except OSError:
+        m = None
+    if not m:
+        return None, "no counter summary"
+    return float(m.group(1)), (f"{TILE_PROFILE.relative_to(ROOT)}: rocprofv3 --pmc FETCH_SIZE x 2.00 + WRITE_SIZE "
+                               "x 1.00 per dispatch (1 GiB copy calibration)")
 
 
 def adjacent_ordered_pairs(cells, nv):
@@ -48,6 +62,8 @@ def cpu_baseline(points, cells, k, target_s, threads):
     import numpy as np
 
     import oracle_lib as O
+
+    O.use_native_build()  # the AVX-512 build on hosts that have it (as target-cpu=native would)
 
     n = len(cells)
     rows = 64
@@ -205,11 +221,10 @@ def config_suite(steps, warmup):
             fp64_diff = float(diff)
             del ref64
         rows = min(32, n)
-        err = None  # the oracle restricts rows for P0 test spaces only; P1 parity is in the GPU tests
-        if sp == "dp0":
-            ref = O.assemble(kind, op, g.points, g.cells, k=k, rows=(0, rows))
-            got = out[:rows].cpu().numpy().astype(ref.dtype)
-            err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref)) if np.linalg.norm(ref) > 0 else 0.0
+        deg = (1, True) if sp == "p1" else (0, False)
+        ref = O.assemble(kind, op, g.points, g.cells, test=deg, trial=deg, k=k, rows=(0, rows))
+        got = out[:rows].cpu().numpy().astype(ref.dtype)
+        err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref)) if np.linalg.norm(ref) > 0 else 0.0
         line = {"config": f"{kind}_{op}_{sp}_icosphere_l{level}" + (f"_k{k:g}" if kind == "helmholtz" else ""),
                 "precision": prec, "output": np.dtype(dt).name, "dofs": n, "entries": n * n, "ms_per_assembly": ms,
                 "entries_per_s": n * n / (ms * 1e-3),
@@ -272,6 +287,13 @@ def main():
     stream = torch.cuda.current_stream()
     print(f"[bench] icosphere L{args.level}: {n} cells, {n * n} entries; warmup {args.warmup}", file=sys.stderr,
           flush=True)
+    # the first call on a new grid also builds its device data on the host and uploads it
+    # (geometry, rule points, adjacent pairs, tile list): the cold first-call time
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    asm.assemble_into_device(space, space, out, stream=stream)
+    torch.cuda.synchronize()
+    cold_ms = (time.perf_counter() - t0) * 1e3
     for _ in range(args.warmup):
         asm.assemble_into_device(space, space, out, stream=stream)
     torch.cuda.synchronize()
@@ -311,6 +333,11 @@ def main():
         masm = create_helmholtz_assembler(args.k, OperatorType.SingleLayer, mopts, dtype=mdt)
         masm.set_profiling(True)
         mout = out if mdt == np.complex128 else torch.empty((n, n), dtype=tdt, device="cuda").t()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        masm.assemble_into_device(space, space, mout, stream=stream)
+        torch.cuda.synchronize()
+        mcold = (time.perf_counter() - t0) * 1e3  # first mixed call: builds the FP32 records and near pairs
         for _ in range(args.warmup):
             masm.assemble_into_device(space, space, mout, stream=stream)
         torch.cuda.synchronize()
@@ -327,7 +354,7 @@ def main():
         key = "complex64" if mdt == np.complex64 else "complex128"
         mixed[key] = {"value": entries / (mms * 1e-3), "ms_per_step": mms, "speedup_vs_fp64": ms_per_step / mms,
                       "sample_rel_err_vs_oracle": merr, "regular_kernels_ms": float(np.mean(mreg)),
-                      "adjacent_kernel_ms": float(np.mean(msing)),
+                      "adjacent_kernel_ms": float(np.mean(msing)), "first_call_ms": mcold,
                       "written_bytes_per_entry": 8 if mdt == np.complex64 else 16}
         print(f"[bench] mixed {key}: {mms:.3f} ms/step, far+near kernels {np.mean(mreg):.3f} ms, adjacent "
               f"{np.mean(msing):.3f} ms, sample rel err {merr:.2e}", file=sys.stderr, flush=True)
@@ -368,10 +395,11 @@ def main():
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         v, crow, ct = cpu_baseline(grid.points, grid.cells, args.k, args.cpu_seconds, threads)
-        cpu = {"value": v, "unit": "entries/s", "cores": threads, "kind": "port",
+        cpu = {"value": v, "unit": "entries/s", "cores": threads, "kind": "port", "isa": O.isa(),
                "sample": f"rows 0..{crow} of the same operator ({crow}x{n} entries incl. their Duffy pairs) "
-                         f"in {ct:.1f} s, oracle with AVX2 libmvec sin/cos, OpenMP {threads} threads"}
+                         f"in {ct:.1f} s, oracle built {O.isa()}, OpenMP {threads} threads"}
 
+    traffic, traffic_src = profiled_traffic()
     line = {
         "metric": "assembled entries/s, Helmholtz SL P0 20480 tris: fp64 vs mixed @<=1e-6 rel err",
         "value": value, "unit": "entries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -382,12 +410,17 @@ def main():
                    "output": "device-resident complex128 column-major"},
         "sample_rel_err_vs_oracle": sample_err,
         "mixed": dict(mixed["complex64"], output="complex64", complex128_output=mixed["complex128"],
-                      scheme="FP32 far pairs (R >= 3 (rad_i + rad_j)) on centroid offsets with FP64 cell-pair "
-                             "phase reduction; near and adjacent pairs and all accumulation FP64"),
+                      scheme="far cell pairs (R >= 3 (rad_i + rad_j)) in FP32: point pairs from FP32 centroid "
+                             "offsets, summed in FP32 per weight class, FP32 per-cell-pair tail (class weights, "
+                             "length scaling, rotation by e^{ikR}); FP64 centroid distance R and reduction of kR; "
+                             "near non-adjacent and adjacent (Duffy) pairs in FP64"),
+        "cold_first_call_ms": cold_ms,
+        "setup_ms": cold_ms - ms_per_step,
         "kernels_ms": {"regular": float(np.mean(reg_ms)), "adjacent": float(np.mean(sing_ms))},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": TILE_KERNEL_TRAFFIC_L5_K5 if (args.level, args.k) == (5, 5.0) else None,
+                     "traffic": traffic if (args.level, args.k) == (5, 5.0) else None,
+                     "traffic_source": traffic_src if (args.level, args.k) == (5, 5.0) else None,
                      "hbm_write_GBps": write_gbps, "hbm_write_frac": write_gbps / HBM_PEAK_GBPS,
                      "basis": f"{n_reg} regular pairs x (36 x {FLOP_PER_QPP} + {FLOP_PER_PAIR_EPILOGUE}) nominal "
                               "flop per launch / mean regular-kernel HIP-event time"},

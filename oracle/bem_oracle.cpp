@@ -51,7 +51,7 @@ constexpr double kPi = 3.14159265358979323846;
 // ---------------------------------------------------------------------------
 // Meshes: src/shapes.rs:19-114. Each refinement splits every triangle at its
 // normalised edge midpoints; children [c0,e2,e1],[c1,e0,e2],[c2,e1,e0],[e0,e1,e2]
-// with e0 = mid(c1,c2), e1 = mid(c0,c2), e2 = mid(c0,c1) (shapes.rs:255-285).
+// with e0 = mid(c1,c2), e1 = mid(c0,c2), e2 = mid(c0,c1) (src/shapes.rs:59-101, children at :95-98).
 // ---------------------------------------------------------------------------
 struct Mesh {
   std::vector<double> pts;     // [nv][3]
@@ -93,7 +93,7 @@ Mesh refine_sphere(std::vector<double> pts, std::vector<int64_t> cells, int leve
   return Mesh{pts, cells};
 }
 
-// Octahedron seed of src/shapes.rs:225-243.
+// Octahedron seed of src/shapes.rs:38-56.
 Mesh octahedron_sphere(int level) {
   std::vector<double> p = {0, 0, 1, 1, 0, 0, 0, 1, 0, -1, 0, 0, 0, -1, 0, 0, 0, -1};
   std::vector<int64_t> c = {0, 1, 2, 0, 2, 3, 0, 3, 4, 0, 4, 1, 5, 2, 1, 5, 3, 2, 5, 4, 3, 5, 1, 4};
@@ -437,8 +437,14 @@ inline void map_point(const CellGeom& G, double x0, double x1, double* out) {
 //   Laplace:   G = 1/(4 pi r),        grad_y G = (x-y) / (4 pi r^3)
 //   Helmholtz: G = e^{ikr}/(4 pi r),  grad_y G = e^{ikr} (1 - ikr) (x-y) / (4 pi r^3)
 // ---------------------------------------------------------------------------
+// glibc libmvec vector sin/cos: 8-wide AVX-512 when built for it (-march=x86-64-v4), else 4-wide AVX2
+#if defined(__AVX512F__)
+extern "C" __m512d _ZGVeN8v_sin(__m512d);
+extern "C" __m512d _ZGVeN8v_cos(__m512d);
+#else
 extern "C" __m256d _ZGVdN4v_sin(__m256d);
 extern "C" __m256d _ZGVdN4v_cos(__m256d);
+#endif
 
 struct KernelSpec {
   bool helmholtz;
@@ -480,12 +486,21 @@ void eval_kernel(const KernelSpec& K, const double* xs, int np, const double* ys
   if (K.helmholtz) {
     int i = 0;
     if (K.fast) {
+#if defined(__AVX512F__)
+      const __m512d kk = _mm512_set1_pd(K.k);
+      for (; i + 8 <= n; i += 8) {
+        __m512d kr = _mm512_mul_pd(kk, _mm512_loadu_pd(R + i));
+        _mm512_storeu_pd(S + i, _ZGVeN8v_sin(kr));
+        _mm512_storeu_pd(C + i, _ZGVeN8v_cos(kr));
+      }
+#else
       const __m256d kk = _mm256_set1_pd(K.k);
       for (; i + 4 <= n; i += 4) {
         __m256d kr = _mm256_mul_pd(kk, _mm256_loadu_pd(R + i));
         _mm256_storeu_pd(S + i, _ZGVdN4v_sin(kr));
         _mm256_storeu_pd(C + i, _ZGVdN4v_cos(kr));
       }
+#endif
     }
     for (; i < n; ++i) { S[i] = std::sin(K.k * R[i]); C[i] = std::cos(K.k * R[i]); }
   }
@@ -515,7 +530,7 @@ void eval_kernel(const KernelSpec& K, const double* xs, int np, const double* ys
 // ---------------------------------------------------------------------------
 // Integrands (src/boundary_assemblers/integrands/*.rs). op: 0 SL, 1 DL, 2 ADL, 3 HS.
 // HS is curl-curl for Laplace (src/laplace.rs:65-78) and curl-curl plus
-// (-k^2) x normal-normal for Helmholtz (src/helmholtz.rs:167-186).
+// (-k^2) x normal-normal for Helmholtz (src/helmholtz.rs:89-107).
 // ---------------------------------------------------------------------------
 struct PointData {
   double val, d0, d1;  // basis value and reference derivatives
@@ -781,6 +796,15 @@ Job make_job(int kind, int op, double k, int nq, int nsing, int batch, int nthre
 extern "C" {
 
 const char* oracle_last_error() { return g_err.c_str(); }
+
+// Instruction set the library was built for (the timed baseline reports it).
+const char* oracle_isa() {
+#if defined(__AVX512F__)
+  return "x86-64-v4 (AVX-512, libmvec 8-wide sin/cos)";
+#else
+  return "x86-64-v3 (AVX2+FMA, libmvec 4-wide sin/cos)";
+#endif
+}
 
 // Dense assembly, accumulated into `out` (column-major; double for Laplace,
 // complex<double> interleaved for Helmholtz). Rows restricted to the test DOFs in

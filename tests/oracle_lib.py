@@ -7,22 +7,47 @@ from pathlib import Path
 import numpy as np
 
 ROOT = Path(__file__).resolve().parent.parent
-LIB_PATH = ROOT / "oracle" / "_build" / "libbem_oracle.so"
+# BEMPP_ORACLE_LIBRARY selects another build (e.g. the sanitizer build, make -C oracle sanitize)
+LIB_PATH = (Path(os.environ["BEMPP_ORACLE_LIBRARY"]) if os.environ.get("BEMPP_ORACLE_LIBRARY")
+            else ROOT / "oracle" / "_build" / "libbem_oracle.so")
+# the same oracle built for AVX-512 hosts (the timed CPU baseline)
+LIB_PATH_AVX512 = ROOT / "oracle" / "_build" / "libbem_oracle_avx512.so"
 
 OPS = {"single_layer": 0, "double_layer": 1, "adjoint_double_layer": 2, "hypersingular": 3}
 KINDS = {"laplace": 0, "helmholtz": 1}
 
 _lib = None
+_path = LIB_PATH
+
+
+def host_has_avx512():
+    try:
+        return " avx512f " in " " + Path("/proc/cpuinfo").read_text().replace("\n", " ") + " "
+    except OSError:
+        return False
+
+
+def use_native_build():
+    """Switch to the AVX-512 build when the host supports it and it is built; returns its ISA."""
+    global _lib, _path
+    if host_has_avx512() and LIB_PATH_AVX512.exists():
+        _path, _lib = LIB_PATH_AVX512, None
+    return isa()
+
+
+def isa():
+    return lib().oracle_isa().decode()
 
 
 def lib():
     global _lib
     if _lib is None:
-        if not LIB_PATH.exists():
-            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run __graft_entry__.build())")
-        L = ctypes.CDLL(str(LIB_PATH))
+        if not _path.exists():
+            raise RuntimeError(f"oracle library missing: {_path} (run __graft_entry__.build())")
+        L = ctypes.CDLL(str(_path))
         i64, i32, dbl, vp = ctypes.c_int64, ctypes.c_int, ctypes.c_double, ctypes.c_void_p
         L.oracle_last_error.restype = ctypes.c_char_p
+        L.oracle_isa.restype = ctypes.c_char_p
         L.oracle_assemble_dense.argtypes = [i32, i32, dbl, vp, i64, vp, i64, i32, i32, i32, i32, i32, i32, i32,
                                             i32, i32, i64, i64, vp]
         L.oracle_assemble_singular_coo.argtypes = [i32, i32, dbl, vp, i64, vp, i64, i32, i32, i32, i32, i32, i32,

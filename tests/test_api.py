@@ -225,7 +225,7 @@ def test_handles_held_by_module_globals_exit_cleanly():
     import subprocess
     import sys
 
-    pkg = Path(L.lib_path()).parent.parent
+    pkg = Path(bempp_amd.__file__).resolve().parent.parent
     code = ("import torch\n"
             "from bempp_amd.assembly.boundary import BoundaryAssemblerOptions, OperatorType, "
             "create_helmholtz_assembler\n"

@@ -322,3 +322,28 @@ def test_potential_identities_oracle():
         S = O.potential("laplace", "single_layer", pts, cells, outside).sum(axis=1)
         errs.append(np.abs(S - 1.0 / np.linalg.norm(outside, axis=1)).max())
     assert errs[1] < errs[0] / 3.5  # O(h^2)
+
+
+@pytest.mark.parametrize("kind,op,k", [("helmholtz", "single_layer", 3.0), ("laplace", "double_layer", 0.0),
+                                       ("helmholtz", "hypersingular", 3.0), ("helmholtz", "adjoint_double_layer", 3.0)])
+def test_p1_basis_placement_invariant_under_vertex_relabelling(kind, op, k):
+    """P1 basis functions at the Duffy points follow the cells' vertices: rotating every
+    cell's vertex list cyclically (orientation kept) changes which local vertex each shared
+    vertex is, hence the Sauter-Schwab permutations used, but not the operator. Entries indexed
+    by physical vertex pairs agree to the Duffy quadrature error (10 points per dimension:
+    1e-13 to 2e-12), while a basis function placed at the wrong vertex would change entries by O(1)."""
+    p, c = O.sphere(1, "icosahedron")
+    shift = np.random.default_rng(3).integers(0, 3, len(c))
+    c2 = np.array([np.roll(row, -s) for row, s in zip(c, shift)])
+    out = []
+    for cells in (c, c2):
+        A = O.assemble(kind, op, p, cells, test=(1, True), trial=(1, True), k=k, nsing=10)
+        _, dofs, _, _ = O.space(p, cells, 1, True)
+        vdof = np.zeros(len(p), dtype=np.int64)
+        vdof[cells.ravel()] = dofs.ravel()  # vertex -> DOF
+        out.append(A[np.ix_(vdof, vdof)])  # rows/columns by vertex
+    diff = np.abs(out[0] - out[1]).max() / np.abs(out[0]).max()
+    assert diff < 1e-10
+    # the same check is sensitive: pairing a vertex with its neighbour's row changes entries by O(1)
+    swapped = out[1][np.roll(np.arange(len(p)), 1)]
+    assert np.abs(out[0] - swapped).max() / np.abs(out[0]).max() > 1e-2
