@@ -1010,11 +1010,21 @@ void kernel_times(const Assembler& a, float* ms) {
   }
 }
 
+// Refuses a device staging buffer larger than the free device memory with a clear error.
+static void check_device_memory(size_t bytes, const char* what) {
+  size_t free_b = 0, total_b = 0;
+  HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+  if (bytes > free_b)
+    throw Error(std::string(what) + " needs " + std::to_string(bytes) + " bytes of device memory, " +
+                std::to_string(free_b) + " of " + std::to_string(total_b) + " are free");
+}
+
 void assemble_host(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test, void* host_out) {
   if (!host_out) throw Error("output pointer is null");
   const int device = current_device();
   (void)device;
   const size_t bytes = (size_t)test.size * (size_t)trial.size * dtype_size(a.dtype);
+  check_device_memory(bytes, "assembly into host memory (staged on the device)");
   void* d = nullptr;
   HIP_CHECK(hipMalloc(&d, std::max<size_t>(bytes, 1)));
   try {
@@ -1241,6 +1251,7 @@ void assemble_potential_host(const Assembler& a, const FunctionSpace& space, con
   }
   const int device = current_device();
   const size_t bytes = (size_t)npoints * (size_t)space.size * dtype_size(a.dtype);
+  check_device_memory(bytes + (size_t)npoints * 3 * sizeof(double), "potential assembly into host memory");
   void* d = nullptr;
   double* dp = nullptr;
   HIP_CHECK(hipMalloc(&d, std::max<size_t>(bytes, 1)));

@@ -531,125 +531,6 @@ __global__ void __launch_bounds__(256) regular_kernel(const RegularArgs A) {
   }
 }
 
-// Adds a tile's LDS block acc[trial DOF][test DOF] (block-local indices) to the output
-// with global atomics. A mirrored tile (symmetric operator, off-diagonal tile) also adds
-// the transpose, in a second pass whose consecutive threads take consecutive trial DOFs:
-// the transposed entries of one test DOF are then neighbours in memory, so a wave's
-// atomics fall into a few cache lines instead of one line per lane.
-template <typename Args>
-__device__ __forceinline__ void flush_tile(const Args& A, const double2* acc, int t0, int ndt, int s0, int nds,
-                                           bool mirror) {
-  const int nacc = ndt * nds;
-  for (int e = threadIdx.x; e < nacc; e += blockDim.x) {
-    const double2 v = acc[e];
-    if (v.x == 0.0 && v.y == 0.0) continue;
-    const int lsi = e / ndt, lti = e - lsi * ndt;
-    const long long row = A.test_blk_dofs[t0 + lti], col = A.trial_blk_dofs[s0 + lsi];
-    store_value(A.out, row + A.ld * col, {v.x, v.y}, A.out_type, kAtomic);
-  }
-  if (!mirror) return;
-  for (int e = threadIdx.x; e < nacc; e += blockDim.x) {
-    const int lti = e / nds, lsi = e - lti * nds;
-    const double2 v = acc[lsi * ndt + lti];
-    if (v.x == 0.0 && v.y == 0.0) continue;
-    const long long row = A.test_blk_dofs[t0 + lti], col = A.trial_blk_dofs[s0 + lsi];
-    store_value(A.out, col + A.ld * row, {v.x, v.y}, A.out_type, kAtomic);
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Regular pairs with a P1 side, tiled: one workgroup (4 wavefronts) per tile of 64 test
-// x 64 trial cells (lane = test cell, 16 trial cells per wavefront). The local blocks of
-// the tile's cell pairs are summed in LDS at (block-local test DOF, block-local trial DOF)
-// -- a compact 64-cell patch of a P1 space has ~45 vertices, so the tile touches ~45 x 45
-// entries instead of 64 x 64 x 9 -- and each non-zero LDS entry is then added to the
-// output with one global atomic. symmetric (single layer / hypersingular on one space):
-// tiles a <= b only; an off-diagonal tile also adds its entries at the transposed place.
-// Dynamic LDS: (max block DOFs)^2 double2.
-// ---------------------------------------------------------------------------
-template <int Q, int OP, bool HELM, int CENTRED>
-__global__ void __launch_bounds__(256) regular_tiled_lds(const RegularArgs A, const int2* __restrict__ tiles,
-                                                         int symmetric) {
-  extern __shared__ double2 acc[];
-  const int2 tile = tiles[blockIdx.x];
-  const int t0 = A.test_blk_off[tile.x], ndt = A.test_blk_off[tile.x + 1] - t0;
-  const int s0 = A.trial_blk_off[tile.y], nds = A.trial_blk_off[tile.y + 1] - s0;
-  const int nacc = ndt * nds;
-  for (int e = threadIdx.x; e < nacc; e += blockDim.x) acc[e] = make_double2(0.0, 0.0);
-  __syncthreads();
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int i = tile.x * kTile + lane;
-  const bool i_ok = i < A.n_test;
-  const int il = i_ok ? i : A.n_test - 1;
-  // Helmholtz: test points read per point (fewer registers, more waves); Laplace: held in
-  // registers (its kernel is short enough that the register version runs at higher occupancy)
-  constexpr bool GEN = Q == 0;  // rule of any point count, points streamed
-  constexpr int QR = GEN ? 1 : Q;
-  const double* tpts = A.test_pts + (size_t)il * (GEN ? A.nq : Q) * 4;
-  double xp[QR][3], wp[QR];
-  if constexpr (!HELM && !GEN) {
-#pragma unroll
-    for (int p = 0; p < Q; ++p) {
-      xp[p][0] = tpts[4 * p];
-      xp[p][1] = tpts[4 * p + 1];
-      xp[p][2] = tpts[4 * p + 2];
-      wp[p] = tpts[4 * p + 3];
-    }
-  }
-  const double* tgeo = A.test_geo + (size_t)il * kGeoStride;
-  const double nx[3] = {tgeo[9], tgeo[10], tgeo[11]};
-  const double ct[3] = {tgeo[kGeoCentroid], tgeo[kGeoCentroid + 1], tgeo[kGeoCentroid + 2]};
-  const int tv0 = A.test_vid[4 * il], tv1 = A.test_vid[4 * il + 1], tv2 = A.test_vid[4 * il + 2];
-  int lt[3] = {0, 0, 0};
-#pragma unroll
-  for (int a = 0; a < 3; ++a)
-    if (a < A.nbt) lt[a] = A.test_ldof[(size_t)il * A.nbt + a];
-  const auto trial_pts = constant_view(A.trial_pts);
-  const auto trial_geo = constant_view(A.trial_geo);
-  const auto trial_vid = constant_view(A.trial_vid);
-  const auto trial_ldof = constant_view(A.trial_ldof);
-#pragma unroll 1
-  for (int jj = 0; jj < 16; ++jj) {
-    const int j = tile.y * kTile + wave * 16 + jj;
-    if (j >= A.n_trial) break;  // wave-uniform
-    const auto spp = trial_pts + (size_t)j * (GEN ? A.nq : Q) * 4;
-    double sp[QR * 4];
-    if constexpr (!GEN) {
-#pragma unroll
-      for (int m = 0; m < Q * 4; ++m) sp[m] = spp[m];
-    }
-    const auto sg = trial_geo + (size_t)j * kGeoStride;
-    const int sv0 = trial_vid[4 * j], sv1 = trial_vid[4 * j + 1], sv2 = trial_vid[4 * j + 2];
-    int ls[3] = {0, 0, 0};
-#pragma unroll
-    for (int b = 0; b < 3; ++b)
-      if (b < A.nbs) ls[b] = trial_ldof[(size_t)j * A.nbs + b];
-    const bool adjacent = A.same_grid && (tv0 == sv0 || tv0 == sv1 || tv0 == sv2 || tv1 == sv0 || tv1 == sv1 ||
-                                          tv1 == sv2 || tv2 == sv0 || tv2 == sv1 || tv2 == sv2);
-    if (adjacent || !i_ok) continue;
-    cplx M[3][3], msum;
-    if constexpr (GEN)
-      p1_local_generic<OP, HELM, CENTRED>(A, tpts, nx, ct, spp, sg, M, msum);
-    else if constexpr (HELM)
-      p1_local_streamed<Q, OP, HELM, CENTRED>(A, tpts, nx, ct, sp, sg, M, msum);
-    else
-      p1_local<Q, OP, HELM, CENTRED>(A, xp, wp, nx, ct, sp, sg, M, msum);
-#pragma unroll
-    for (int b = 0; b < 3; ++b)
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        if (a >= A.nbt || b >= A.nbs) continue;  // uniform; constant indices keep M in registers
-        const cplx v = local_entry<OP, HELM>(M, a, b, msum, A.nbt, A.nbs, tgeo, sg, a, b, A.k);
-        double2* slot = acc + (ls[b] * ndt + lt[a]);
-        atomicAdd(&slot->x, v.re);
-        if constexpr (HELM) atomicAdd(&slot->y, v.im);
-      }
-  }
-  __syncthreads();
-  flush_tile(A, acc, t0, ndt, s0, nds, symmetric && tile.x != tile.y);
-}
-
 // Weight classes of the symmetric triangle rules (triangle_rule): the points of one orbit
 // share a weight, so the mixed-precision far-field point-pair sums are taken per (test
 // class, trial class) in FP32, unweighted, and weighted once per cell pair. Q = 1, 3: one class;
@@ -676,18 +557,9 @@ __host__ __device__ constexpr double orbit_t(int Q, int q) {
 }
 __host__ __device__ constexpr int orbit_vertex(int Q, int q) { return Q == 7 ? (q == 0 ? 0 : (q - 1) % 3) : q % 3; }
 
-// ---------------------------------------------------------------------------
-// P0 x P0 regular pairs, tiled. One workgroup (4 wavefronts) per 64 x 64 tile
-// (tile row a: test cells 64a + lane; tile column b: trial cells, 16 per
-// wavefront in two chunks of 8). The tile is written column-major with
-// coalesced stores along the test index. When `symmetric` is set (single
-// layer / hypersingular with identical test and trial space), only tiles with
-// a <= b are launched and an off-diagonal tile is also written transposed at
-// (b, a): A[j][i] sums the same quadrature terms as A[i][j] because G is
-// symmetric and both cells use the same rule. The transpose goes through LDS
-// so the mirrored stores are 128-byte contiguous segments.
-// ---------------------------------------------------------------------------
-
+// One regular P0 cell pair (any operator; the hypersingular one is the single-layer sum)
+// from point coordinates: test points xp (w_p J_x in wp) held in registers, trial points
+// sp (x, y, z, w_q J_y per point). The near-pair kernels and the matrix-free apply use it.
 template <int Q, int OP, bool HELM, int CENTRED>
 __device__ __forceinline__ cplx p0_pair(const double (&xp)[Q][3], const double (&wp)[Q], const double* nx,
                                         const double (&sp)[Q * 4], const double* ny, double k, double kR) {
@@ -800,6 +672,175 @@ __device__ __forceinline__ cplx p0_pair_orbit(const double (&td)[2][3], const do
   }
   return acc;
 }
+
+// Adds a tile's LDS block acc[trial DOF][test DOF] (block-local indices) to the output
+// with global atomics. A mirrored tile (symmetric operator, off-diagonal tile) also adds
+// the transpose, in a second pass whose consecutive threads take consecutive trial DOFs:
+// the transposed entries of one test DOF are then neighbours in memory, so a wave's
+// atomics fall into a few cache lines instead of one line per lane.
+template <typename Args>
+__device__ __forceinline__ void flush_tile(const Args& A, const double2* acc, int t0, int ndt, int s0, int nds,
+                                           bool mirror) {
+  const int nacc = ndt * nds;
+  for (int e = threadIdx.x; e < nacc; e += blockDim.x) {
+    const double2 v = acc[e];
+    if (v.x == 0.0 && v.y == 0.0) continue;
+    const int lsi = e / ndt, lti = e - lsi * ndt;
+    const long long row = A.test_blk_dofs[t0 + lti], col = A.trial_blk_dofs[s0 + lsi];
+    store_value(A.out, row + A.ld * col, {v.x, v.y}, A.out_type, kAtomic);
+  }
+  if (!mirror) return;
+  for (int e = threadIdx.x; e < nacc; e += blockDim.x) {
+    const int lti = e / nds, lsi = e - lti * nds;
+    const double2 v = acc[lsi * ndt + lti];
+    if (v.x == 0.0 && v.y == 0.0) continue;
+    const long long row = A.test_blk_dofs[t0 + lti], col = A.trial_blk_dofs[s0 + lsi];
+    store_value(A.out, col + A.ld * row, {v.x, v.y}, A.out_type, kAtomic);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Regular pairs with a P1 side, tiled: one workgroup (4 wavefronts) per tile of 64 test
+// x 64 trial cells (lane = test cell, 16 trial cells per wavefront). The local blocks of
+// the tile's cell pairs are summed in LDS at (block-local test DOF, block-local trial DOF)
+// -- a compact 64-cell patch of a P1 space has ~45 vertices, so the tile touches ~45 x 45
+// entries instead of 64 x 64 x 9 -- and each non-zero LDS entry is then added to the
+// output with one global atomic. symmetric (single layer / hypersingular on one space):
+// tiles a <= b only; an off-diagonal tile also adds its entries at the transposed place.
+// Dynamic LDS: (max block DOFs)^2 double2.
+// ---------------------------------------------------------------------------
+template <int Q, int OP, bool HELM, int CENTRED>
+__global__ void __launch_bounds__(256) regular_tiled_lds(const RegularArgs A, const int2* __restrict__ tiles,
+                                                         int symmetric) {
+  extern __shared__ double2 acc[];
+  const int2 tile = tiles[blockIdx.x];
+  const int t0 = A.test_blk_off[tile.x], ndt = A.test_blk_off[tile.x + 1] - t0;
+  const int s0 = A.trial_blk_off[tile.y], nds = A.trial_blk_off[tile.y + 1] - s0;
+  const int nacc = ndt * nds;
+  for (int e = threadIdx.x; e < nacc; e += blockDim.x) acc[e] = make_double2(0.0, 0.0);
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i = tile.x * kTile + lane;
+  const bool i_ok = i < A.n_test;
+  const int il = i_ok ? i : A.n_test - 1;
+  // Helmholtz: test points read per point (fewer registers, more waves); Laplace: held in
+  // registers (its kernel is short enough that the register version runs at higher occupancy)
+  constexpr bool GEN = Q == 0;  // rule of any point count, points streamed
+  constexpr int QR = GEN ? 1 : Q;
+  const double* tpts = A.test_pts + (size_t)il * (GEN ? A.nq : Q) * 4;
+  double xp[QR][3], wp[QR];
+  if constexpr (!HELM && !GEN) {
+#pragma unroll
+    for (int p = 0; p < Q; ++p) {
+      xp[p][0] = tpts[4 * p];
+      xp[p][1] = tpts[4 * p + 1];
+      xp[p][2] = tpts[4 * p + 2];
+      wp[p] = tpts[4 * p + 3];
+    }
+  }
+  const double* tgeo = A.test_geo + (size_t)il * kGeoStride;
+  const double nx[3] = {tgeo[9], tgeo[10], tgeo[11]};
+  const double ct[3] = {tgeo[kGeoCentroid], tgeo[kGeoCentroid + 1], tgeo[kGeoCentroid + 2]};
+  const int tv0 = A.test_vid[4 * il], tv1 = A.test_vid[4 * il + 1], tv2 = A.test_vid[4 * il + 2];
+  // Laplace hypersingular: the P1 surface curls are constant on a cell, so a cell pair needs
+  // only sum_pq (w_p J_x)(w_q J_y) G_pq -- the single-layer P0 sum, taken in orbit form as
+  // the P0 tiles take it (p0_pair_orbit) when the rule has that form
+  constexpr bool LHS = OP == 3 && !HELM && !GEN;
+  const bool lhs_orbit = LHS && A.orbit_form;
+  double td[2][3], tau2g[QR], twc[3];
+  if constexpr (LHS) {
+    if (lhs_orbit) {
+      const double* tv = A.test_vtx64 + (size_t)il * 12;
+#pragma unroll
+      for (int m = 0; m < 2; ++m)
+        for (int d = 0; d < 3; ++d) td[m][d] = tv[4 * m + d];
+      const double tg[3] = {tv[3], tv[7], tv[11]};
+#pragma unroll
+      for (int p = 0; p < QR; ++p) {
+        const double tau = orbit_t(Q, p);
+        tau2g[p] = tau * tau * tg[orbit_vertex(Q, p)];
+      }
+      for (int c = 0; c < 3; ++c) twc[c] = tv[8 + c];
+    }
+  }
+  const auto trial_vtx = constant_view(A.trial_vtx64);
+  int lt[3] = {0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    if (a < A.nbt) lt[a] = A.test_ldof[(size_t)il * A.nbt + a];
+  const auto trial_pts = constant_view(A.trial_pts);
+  const auto trial_geo = constant_view(A.trial_geo);
+  const auto trial_vid = constant_view(A.trial_vid);
+  const auto trial_ldof = constant_view(A.trial_ldof);
+#pragma unroll 1
+  for (int jj = 0; jj < 16; ++jj) {
+    const int j = tile.y * kTile + wave * 16 + jj;
+    if (j >= A.n_trial) break;  // wave-uniform
+    const auto spp = trial_pts + (size_t)j * (GEN ? A.nq : Q) * 4;
+    double sp[QR * 4];
+    if constexpr (!GEN) {
+#pragma unroll
+      for (int m = 0; m < Q * 4; ++m) sp[m] = spp[m];
+    }
+    const auto sg = trial_geo + (size_t)j * kGeoStride;
+    const int sv0 = trial_vid[4 * j], sv1 = trial_vid[4 * j + 1], sv2 = trial_vid[4 * j + 2];
+    int ls[3] = {0, 0, 0};
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+      if (b < A.nbs) ls[b] = trial_ldof[(size_t)j * A.nbs + b];
+    const bool adjacent = A.same_grid && (tv0 == sv0 || tv0 == sv1 || tv0 == sv2 || tv1 == sv0 || tv1 == sv1 ||
+                                          tv1 == sv2 || tv2 == sv0 || tv2 == sv1 || tv2 == sv2);
+    if (adjacent || !i_ok) continue;
+    cplx M[3][3], msum;
+    if (LHS && lhs_orbit) {
+      if constexpr (LHS) {
+        const auto svp = trial_vtx + (size_t)j * 12;
+        double sv[12];
+#pragma unroll
+        for (int m = 0; m < 12; ++m) sv[m] = svp[m];
+        const double D[3] = {ct[0] - sg[kGeoCentroid], ct[1] - sg[kGeoCentroid + 1], ct[2] - sg[kGeoCentroid + 2]};
+        const double R2 = D[0] * D[0] + D[1] * D[1] + D[2] * D[2];
+        const double ny[3] = {sg[9], sg[10], sg[11]};
+        msum = p0_pair_orbit<Q, 0, false, 0>(td, tau2g, twc, sv, D, R2, nx, ny, 0.0, 0.0);
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+          for (int b = 0; b < 3; ++b) M[a][b] = {0.0, 0.0};  // unused by the curl-curl entries
+      }
+    } else if constexpr (GEN) {
+      p1_local_generic<OP, HELM, CENTRED>(A, tpts, nx, ct, spp, sg, M, msum);
+    } else if constexpr (HELM) {
+      p1_local_streamed<Q, OP, HELM, CENTRED>(A, tpts, nx, ct, sp, sg, M, msum);
+    } else {
+      p1_local<Q, OP, HELM, CENTRED>(A, xp, wp, nx, ct, sp, sg, M, msum);
+    }
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        if (a >= A.nbt || b >= A.nbs) continue;  // uniform; constant indices keep M in registers
+        const cplx v = local_entry<OP, HELM>(M, a, b, msum, A.nbt, A.nbs, tgeo, sg, a, b, A.k);
+        double2* slot = acc + (ls[b] * ndt + lt[a]);
+        atomicAdd(&slot->x, v.re);
+        if constexpr (HELM) atomicAdd(&slot->y, v.im);
+      }
+  }
+  __syncthreads();
+  flush_tile(A, acc, t0, ndt, s0, nds, symmetric && tile.x != tile.y);
+}
+
+// ---------------------------------------------------------------------------
+// P0 x P0 regular pairs, tiled. One workgroup (4 wavefronts) per 64 x 64 tile
+// (tile row a: test cells 64a + lane; tile column b: trial cells, 16 per
+// wavefront in two chunks of 8). The tile is written column-major with
+// coalesced stores along the test index. When `symmetric` is set (single
+// layer / hypersingular with identical test and trial space), only tiles with
+// a <= b are launched and an off-diagonal tile is also written transposed at
+// (b, a): A[j][i] sums the same quadrature terms as A[i][j] because G is
+// symmetric and both cells use the same rule. The transpose goes through LDS
+// so the mirrored stores are 128-byte contiguous segments.
+// ---------------------------------------------------------------------------
 
 // One regular P0 cell pair for a rule of any point count (nq): the lane's test points
 // streamed from tpts (x, y, z, w J per point), the wave-uniform trial cell's points read

@@ -139,6 +139,53 @@ def batch_leg(seed, n, steps, mixed):
     return entries, res
 
 
+def shape_gradient_leg(level, warmup, steps):
+    """Times the matrix-free shape gradient (regular-pair and adjacent-pair adjoint kernels and
+    the vertex scatter) with device inputs; one step is one full gradient."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    from bempp_amd import _lib as L
+    from bempp_amd.assembly.boundary import BoundaryAssemblerOptions, OperatorType, create_helmholtz_assembler
+    from bempp_amd.element import Continuity, Family, ReferenceCellType, create_family
+    from bempp_amd.function_space import function_space
+    from bempp_amd.grid import Grid
+    from bempp_amd.shapes import icosphere
+    from bempp_amd.workloads import complex_normal, perturb_radially
+
+    g0 = icosphere(level)
+    g = Grid(perturb_radially(g0.points, 0), g0.cells)
+    s = function_space(g, create_family(Family.Lagrange, 0, Continuity.Discontinuous))
+    o = BoundaryAssemblerOptions()
+    o.set_regular_quadrature_degree(ReferenceCellType.Triangle, 6)
+    o.set_singular_quadrature_degree(ReferenceCellType.Triangle, ReferenceCellType.Triangle, 4)
+    a = create_helmholtz_assembler(3.0, OperatorType.SingleLayer, o)
+    rng = np.random.default_rng(0)
+    u, v = complex_normal(len(g.cells), rng), complex_normal(len(g.cells), rng)
+    ref = a.shape_gradient(s, s, u, v)  # the allocating entry point (also builds the device data)
+    du, dv = torch.from_numpy(u).cuda(), torch.from_numpy(v).cuda()
+    grad = torch.empty((len(g.points), 3), dtype=torch.complex128, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    args = (a._rs_assembler, s._rs_space, s._rs_space, ctypes.c_void_p(du.data_ptr()), ctypes.c_void_p(dv.data_ptr()),
+            ctypes.c_void_p(grad.data_ptr()), ctypes.c_void_p(stream))
+    for _ in range(max(1, warmup // 4)):
+        L.check(L.lib().boundary_assembler_shape_gradient_noalloc(*args))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        L.check(L.lib().boundary_assembler_shape_gradient_noalloc(*args))
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    diff = float(np.linalg.norm(grad.cpu().numpy() - ref) / np.linalg.norm(ref))
+    n = len(g.cells)
+    print(f"[bench] shape gradient L{level}: {ms:.3f} ms", file=sys.stderr, flush=True)
+    return {"workload": f"d(v^T A(X) u)/dX, Helmholtz SL P0, k = 3, level-{level} icosphere perturbed radially by "
+                        "0.05 sum_{l<=4} a_lm Y_lm (seed 0)", "cells": n, "ms_per_gradient": ms,
+            "cell_pairs_per_s": n * n / (ms * 1e-3), "rel_diff_vs_allocating_entry_point": diff}
+
+
 def copy_calibration():
     """A plain device copy of 1 GiB (read 1 GiB + write 1 GiB per dispatch), three times:
     the reference point that scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -153,7 +200,16 @@ def copy_calibration():
     print(json.dumps({"copy_calibration_bytes_per_dispatch": n * 8, "dispatches": 3}))
 
 
-def config_suite(steps, warmup):
+# nominal flop per regular cell pair with the 6-point rule (36 point pairs), by (kernel,
+# operator, space): the counting rule of FLOP_PER_QPP (+ - x = 1, FMA = 2, sqrt, div, sin,
+# cos = 1) applied to each integrand and its basis products
+NOMINAL_FLOP_PER_PAIR = {("laplace", "single_layer", "dp0"): 470, ("helmholtz", "single_layer", "dp0"): 724,
+                         ("laplace", "double_layer", "p1"): 1485, ("laplace", "adjoint_double_layer", "p1"): 1485,
+                         ("laplace", "hypersingular", "p1"): 524, ("helmholtz", "double_layer", "p1"): 2646,
+                         ("helmholtz", "adjoint_double_layer", "p1"): 2646, ("helmholtz", "hypersingular", "p1"): 2408}
+
+
+def config_suite(steps, warmup, only=None):
     """Every dense configuration of the workload set, timed on the GPU (HIP events around the
     kernels, one launch sequence per step, output in HBM) with 32 sample rows checked against
     the FP64 oracle: one JSON line per configuration."""
@@ -185,6 +241,9 @@ def config_suite(steps, warmup):
              ("laplace", "single_layer", "dp0", 6, 0.0, "fp64", np.float64)]
     grids = {}
     for kind, op, sp, level, k, prec, dt in cases:
+        name = f"{kind}_{op}_{sp}_icosphere_l{level}" + (f"_k{k:g}" if kind == "helmholtz" else "")
+        if only and not any(o in f"{name}_{prec}" for o in only.split(",")):
+            continue
         g = grids.setdefault(level, icosphere(level))
         space = function_space(g, create_family(Family.Lagrange, 1 if sp == "p1" else 0,
                                                 Continuity.Standard if sp == "p1" else Continuity.Discontinuous))
@@ -225,10 +284,17 @@ def config_suite(steps, warmup):
         ref = O.assemble(kind, op, g.points, g.cells, test=deg, trial=deg, k=k, rows=(0, rows))
         got = out[:rows].cpu().numpy().astype(ref.dtype)
         err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref)) if np.linalg.norm(ref) > 0 else 0.0
-        line = {"config": f"{kind}_{op}_{sp}_icosphere_l{level}" + (f"_k{k:g}" if kind == "helmholtz" else ""),
+        reg_ms = float(np.median([t[0] for t in times]))
+        nreg = len(g.cells) ** 2 - adjacent_ordered_pairs(g.cells, len(g.points))
+        nominal = NOMINAL_FLOP_PER_PAIR.get((kind, op, sp))
+        peak = FP64_PEAK_TFLOPS if prec == "fp64" else FP32_PEAK_TFLOPS
+        rate = nreg * nominal / (reg_ms * 1e-3) / 1e12 if nominal else None
+        line = {"config": name,
                 "precision": prec, "output": np.dtype(dt).name, "dofs": n, "entries": n * n, "ms_per_assembly": ms,
                 "entries_per_s": n * n / (ms * 1e-3),
-                "regular_kernels_ms": float(np.median([t[0] for t in times])),
+                "nominal_tflops": rate, "nominal_frac": rate / peak if rate else None,
+                "nominal_basis": f"{nreg} regular cell pairs x {nominal} flop / regular-kernel time, peak {peak}",
+                "regular_kernels_ms": reg_ms,
                 "adjacent_kernel_ms": float(np.median([t[1] for t in times])),
                 "sample_rows": rows, "sample_rel_err_vs_oracle": err, "rel_diff_vs_fp64_gpu": fp64_diff}
         print(json.dumps(line), flush=True)
@@ -240,7 +306,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=5)  # the first launches after idle run at lower clocks
+    # the first launches after idle run at lower clocks (kernel traces: ~6 dispatches to steady state)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--level", type=int, default=5)
     ap.add_argument("--k", type=float, default=5.0)
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
@@ -251,6 +318,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--configs", action="store_true",
                     help="time every dense configuration (one JSON line each) instead of the metric")
+    ap.add_argument("--only", default=None,
+                    help="with --configs: comma-separated substrings of '<config>_<precision>' to run")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB copy used to calibrate the HBM byte counters")
     args = ap.parse_args()
@@ -258,7 +327,7 @@ def main():
         copy_calibration()
         return
     if args.configs:
-        config_suite(max(1, min(args.steps, 5)), max(1, min(args.warmup, 3)))
+        config_suite(max(1, min(args.steps, 5)), max(1, min(args.warmup, 3)), args.only)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine assembles on one GPU; multi-GPU runs are not part of this build "
@@ -314,11 +383,10 @@ def main():
     print(f"[bench] {ms_per_step:.3f} ms/step, regular kernel {np.mean(reg_ms):.3f} ms, adjacent kernel "
           f"{np.mean(sing_ms):.3f} ms", file=sys.stderr, flush=True)
 
-    # spot check of 64 rows against the oracle
+    # rows kept for the oracle spot check, which runs on the host after all GPU legs (an idle GPU
+    # would drop its clocks before the next leg)
     rows = 64
-    ref = O.assemble("helmholtz", "single_layer", grid.points, grid.cells, k=args.k, fast=False, rows=(0, rows))
     got = out[:rows].cpu().numpy()
-    sample_err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
 
     # mixed precision: FP32 far pairs, FP64 near/adjacent pairs and accumulation; the
     # complex64-mixed form (k = 5 is exact in FP32) and, for comparison, complex128 output
@@ -350,17 +418,29 @@ def main():
             msing.append(s_)
         torch.cuda.synchronize()
         mms = (time.perf_counter() - t0) * 1e3 / args.steps
-        merr = float(np.linalg.norm(mout[:rows].cpu().numpy().astype(np.complex128) - ref) / np.linalg.norm(ref))
+        mgot = mout[:rows].cpu().numpy().astype(np.complex128)
         key = "complex64" if mdt == np.complex64 else "complex128"
         mixed[key] = {"value": entries / (mms * 1e-3), "ms_per_step": mms, "speedup_vs_fp64": ms_per_step / mms,
-                      "sample_rel_err_vs_oracle": merr, "regular_kernels_ms": float(np.mean(mreg)),
+                      "sample_rows": mgot, "regular_kernels_ms": float(np.mean(mreg)),
                       "adjacent_kernel_ms": float(np.mean(msing)), "first_call_ms": mcold,
                       "written_bytes_per_entry": 8 if mdt == np.complex64 else 16}
         print(f"[bench] mixed {key}: {mms:.3f} ms/step, far+near kernels {np.mean(mreg):.3f} ms, adjacent "
-              f"{np.mean(msing):.3f} ms, sample rel err {merr:.2e}", file=sys.stderr, flush=True)
+              f"{np.mean(msing):.3f} ms", file=sys.stderr, flush=True)
         if mout is not out:
             del mout
             torch.cuda.empty_cache()
+
+    # shape gradient d(v^T A(X) u)/dX of the single layer (k = 3) on the level-5 icosphere moved
+    # radially by 0.05 sum a_lm Y_lm (l <= 4, seed 0), u, v complex normal: matrix-free, device
+    # inputs through the entry point that allocates nothing
+    gradient = shape_gradient_leg(args.level, args.warmup, args.steps)
+
+    # the oracle spot check of the sample rows (first 64 rows: test cells 0..63 against all trial cells)
+    ref = O.assemble("helmholtz", "single_layer", grid.points, grid.cells, k=args.k, fast=False, rows=(0, rows))
+    sample_err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    for key, m in mixed.items():
+        m["sample_rel_err_vs_oracle"] = float(np.linalg.norm(m.pop("sample_rows") - ref) / np.linalg.norm(ref))
+        print(f"[bench] mixed {key}: sample rel err {m['sample_rel_err_vs_oracle']:.2e}", file=sys.stderr, flush=True)
 
     n_adj = adjacent_ordered_pairs(grid.cells, len(grid.points))
     n_reg = entries - n_adj
@@ -426,6 +506,7 @@ def main():
                               "flop per launch / mean regular-kernel HIP-event time"},
         "cpu_baseline": cpu,
         "batched": batched,
+        "shape_gradient": gradient,
     }
     print(json.dumps(line))
 

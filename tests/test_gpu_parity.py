@@ -406,3 +406,26 @@ def test_p1_basis_placement_vertex_relabelling(op):
     diff = np.abs(out[0] - out[1]).max() / np.abs(out[0]).max()
     print(f"[placement] {op}: relabelled vs original {diff:.2e}", flush=True)
     assert diff < 1e-10
+
+
+def test_oversized_host_assembly_fails_cleanly():
+    """A dense matrix larger than the free device memory (level-7 icosphere, 327680 cells:
+    859 GB in float64) is refused with an error before anything is staged or copied; the
+    library keeps working afterwards."""
+    import ctypes
+
+    from bempp_amd import _lib as L
+
+    g = icosphere(7)
+    s = _space(g, "dp0")
+    a = _assembler("laplace", "single_layer", 0.0, _options())
+    dummy = np.zeros(1)
+    rc = L.lib().boundary_assembler_assemble_into_memory(a._rs_assembler, s._rs_space, s._rs_space,
+                                                         ctypes.c_void_p(dummy.ctypes.data))
+    assert rc != 0
+    assert "bytes of device memory" in L.last_error()
+    small = icosphere(1)
+    sp = _space(small, "dp0")
+    A = a.assemble(sp, sp)
+    R = O.assemble("laplace", "single_layer", small.points, small.cells)
+    assert _rel(A, R) <= TOL
