@@ -86,6 +86,8 @@ _SIGNATURES = {
     "boundary_assembler_batch_set_graph": (c_int, [c_vp, c_int]),
     "boundary_assembler_batch_set_launch": (c_int, [c_vp, c_int, c_int, c_int]),
     "boundary_assembler_batch_get_launch": (c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "boundary_assembler_batch_set_tile_waves": (c_int, [c_vp, c_int]),
+    "boundary_assembler_batch_get_tile_waves": (c_int, [c_vp, c_vp]),
     "boundary_assembler_batch_counts": (c_int, [c_vp, c_vp]),
     "boundary_assembler_batch_autotune": (ctypes.c_double, [c_vp, ctypes.c_char_p, c_int, c_vp]),
     "boundary_assembler_batch_set_profiling": (c_int, [c_vp, c_int]),

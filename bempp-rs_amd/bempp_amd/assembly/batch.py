@@ -72,14 +72,21 @@ class BatchAssembler:
         """Capture the launch sequence into a hipGraph on the next assemble and replay it."""
         L.check(L.lib().boundary_assembler_batch_set_graph(self._h, 1 if enable else 0))
 
-    def set_launch(self, blocks_per_cu=2, chunk=8, order=0):
+    def set_launch(self, blocks_per_cu=2, chunk=8, order=0, waves=4):
+        """Launch shape: workgroups per CU (0: one per tile), trial chunk (4 or 8), tile order
+        (0 item-major, 1 largest items first), wavefronts per 64 x 64 tile (2, 4 or 8; 2 and
+        8 with chunk 8)."""
+        L.check(L.lib().boundary_assembler_batch_set_tile_waves(self._h, 4))
         L.check(L.lib().boundary_assembler_batch_set_launch(self._h, blocks_per_cu, chunk, order))
+        L.check(L.lib().boundary_assembler_batch_set_tile_waves(self._h, waves))
 
     def launch(self):
-        """(blocks_per_cu, chunk, order) of the current launch shape."""
+        """(blocks_per_cu, chunk, order, waves) of the current launch shape."""
         v = [ctypes.c_int(0) for _ in range(3)]
         L.check(L.lib().boundary_assembler_batch_get_launch(self._h, *[ctypes.byref(x) for x in v]))
-        return tuple(x.value for x in v)
+        w = ctypes.c_int(0)
+        L.check(L.lib().boundary_assembler_batch_get_tile_waves(self._h, ctypes.byref(w)))
+        return tuple(x.value for x in v) + (w.value,)
 
     def counts(self):
         """dict: items, tiles, adjacent pairs, near pairs."""

@@ -11,19 +11,19 @@ so that both draw identical inputs.
 import numpy as np
 
 
-def batch_spec(seed, n):
-    """[(level, semi_axes[3], k)] for n items."""
+def batch_spec(seed, n, levels=(2, 5)):
+    """[(level, semi_axes[3], k)] for n items; level uniform in levels[0]..levels[1]."""
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(n):
-        lev = int(rng.integers(2, 6))
+        lev = int(rng.integers(levels[0], levels[1] + 1))
         axes = rng.uniform(0.5, 2.0, 3)
         k = float(rng.uniform(1.0, 10.0))
         out.append((lev, axes, k))
     return out
 
 
-def batch_items(seed, n, dtype=np.complex128, mixed=False, nq=6, nsing=4):
+def batch_items(seed, n, dtype=np.complex128, mixed=False, nq=6, nsing=4, levels=(2, 5)):
     """Helmholtz single-layer P0 assemblers on the batch_spec meshes: [(grid, space, assembler,
     k, level)]. A complex64 assembler holds k in FP32; the k returned is that FP32 value, so a
     reference assembly with it uses the same wavenumber."""
@@ -33,7 +33,7 @@ def batch_items(seed, n, dtype=np.complex128, mixed=False, nq=6, nsing=4):
     from .shapes import ellipsoid
 
     items = []
-    for lev, axes, k in batch_spec(seed, n):
+    for lev, axes, k in batch_spec(seed, n, levels):
         if np.dtype(dtype) == np.complex64:
             k = float(np.float32(k))
         o = BoundaryAssemblerOptions()

@@ -70,12 +70,13 @@ struct Batch {
   int blocks_per_cu = 2;  // 0: one workgroup per tile
   int chunk = 8;
   int order = 0;
+  int waves = 4;          // wavefronts per 64 x 64 tile (2, 4 or 8)
   // graph
   bool use_graph = false;
   hipStream_t stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  int graph_shape[3] = {-1, -1, -1};
+  int graph_shape[4] = {-1, -1, -1, -1};
   // profiling
   bool profiling = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -103,12 +104,13 @@ void drop_graph(Batch& b) {
   if (b.graph) (void)hipGraphDestroy(b.graph);
   b.exec = nullptr;
   b.graph = nullptr;
-  b.graph_shape[0] = b.graph_shape[1] = b.graph_shape[2] = -1;
+  b.graph_shape[0] = b.graph_shape[1] = b.graph_shape[2] = b.graph_shape[3] = -1;
 }
 
 // ---- launches ----
-template <int Q, int OP, bool HELM, int CH>
+template <int Q, int OP, bool HELM, int CH, int NW>
 void launch_tiles_t(const Batch& b, int blocks, hipStream_t s) {
+  constexpr int kThreads = 64 * NW;
   // group g = phase tier of the items: 0 not centred (range-reduced phase), 1 and 2
   // centred small-argument tiers (Helmholtz only)
   const int sym = b.symmetric ? 1 : 0;
@@ -124,31 +126,34 @@ void launch_tiles_t(const Batch& b, int blocks, hipStream_t s) {
       if constexpr (Q == 0) {
         throw Error("mixed-precision batches need a rule compiled for its point count");
       } else if (!HELM) {
-        dev::batched_p0_mixed_persistent<Q, OP, HELM, 1, CH><<<grid, 256, 0, s>>>(b.items, tg, nt, ctr, sym);
+        dev::batched_p0_mixed_persistent<Q, OP, HELM, 1, CH, NW><<<grid, kThreads, 0, s>>>(b.items, tg, nt, ctr, sym);
       } else if (g == 2) {
-        dev::batched_p0_mixed_persistent<Q, OP, HELM, 2, CH><<<grid, 256, 0, s>>>(b.items, tg, nt, ctr, sym);
+        dev::batched_p0_mixed_persistent<Q, OP, HELM, 2, CH, NW><<<grid, kThreads, 0, s>>>(b.items, tg, nt, ctr, sym);
       } else if (g == 1) {
-        dev::batched_p0_mixed_persistent<Q, OP, HELM, 1, CH><<<grid, 256, 0, s>>>(b.items, tg, nt, ctr, sym);
+        dev::batched_p0_mixed_persistent<Q, OP, HELM, 1, CH, NW><<<grid, kThreads, 0, s>>>(b.items, tg, nt, ctr, sym);
       } else {
-        dev::batched_p0_mixed_persistent<Q, OP, HELM, 0, CH><<<grid, 256, 0, s>>>(b.items, tg, nt, ctr, sym);
+        dev::batched_p0_mixed_persistent<Q, OP, HELM, 0, CH, NW><<<grid, kThreads, 0, s>>>(b.items, tg, nt, ctr, sym);
       }
     } else {
       if (!HELM)
-        dev::batched_p0_persistent<Q, OP, HELM, 0, CH><<<grid, 256, 0, s>>>(b.items, tg, nt, ctr, sym);
+        dev::batched_p0_persistent<Q, OP, HELM, 0, CH, NW><<<grid, kThreads, 0, s>>>(b.items, tg, nt, ctr, sym);
       else if (g == 2)
-        dev::batched_p0_persistent<Q, OP, HELM, 2, CH><<<grid, 256, 0, s>>>(b.items, tg, nt, ctr, sym);
+        dev::batched_p0_persistent<Q, OP, HELM, 2, CH, NW><<<grid, kThreads, 0, s>>>(b.items, tg, nt, ctr, sym);
       else if (g == 1)
-        dev::batched_p0_persistent<Q, OP, HELM, 1, CH><<<grid, 256, 0, s>>>(b.items, tg, nt, ctr, sym);
+        dev::batched_p0_persistent<Q, OP, HELM, 1, CH, NW><<<grid, kThreads, 0, s>>>(b.items, tg, nt, ctr, sym);
       else
-        dev::batched_p0_persistent<Q, OP, HELM, 0, CH><<<grid, 256, 0, s>>>(b.items, tg, nt, ctr, sym);
+        dev::batched_p0_persistent<Q, OP, HELM, 0, CH, NW><<<grid, kThreads, 0, s>>>(b.items, tg, nt, ctr, sym);
     }
   }
 }
 
+// tile shapes: (trial chunk, wavefronts per tile) = (4, 4), (8, 4), (8, 2), (8, 8)
 template <int Q, int OP, bool HELM>
 void launch_tiles_c(const Batch& b, int blocks, hipStream_t s) {
-  if (b.chunk == 4) launch_tiles_t<Q, OP, HELM, 4>(b, blocks, s);
-  else launch_tiles_t<Q, OP, HELM, 8>(b, blocks, s);
+  if (b.waves == 2) launch_tiles_t<Q, OP, HELM, 8, 2>(b, blocks, s);
+  else if (b.waves == 8) launch_tiles_t<Q, OP, HELM, 8, 8>(b, blocks, s);
+  else if (b.chunk == 4) launch_tiles_t<Q, OP, HELM, 4, 4>(b, blocks, s);
+  else launch_tiles_t<Q, OP, HELM, 8, 4>(b, blocks, s);
 }
 
 template <int OP, bool HELM>
@@ -349,8 +354,8 @@ void batch_assemble(Batch& b, void* stream) {
     b.recorded = b.profiling;
     return;
   }
-  const int shape[3] = {b.blocks_per_cu, b.chunk, b.order};
-  if (!b.exec || !std::equal(shape, shape + 3, b.graph_shape)) {
+  const int shape[4] = {b.blocks_per_cu, b.chunk, b.order, b.waves};
+  if (!b.exec || !std::equal(shape, shape + 4, b.graph_shape)) {
     drop_graph(b);
     // capture on the batch's own stream (kernel nodes only), replay on any stream
     hipStream_t cs = own_stream(b);
@@ -365,7 +370,7 @@ void batch_assemble(Batch& b, void* stream) {
     }
     HIP_CHECK_B(hipStreamEndCapture(cs, &b.graph));
     HIP_CHECK_B(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
-    std::copy(shape, shape + 3, b.graph_shape);
+    std::copy(shape, shape + 4, b.graph_shape);
   }
   if (b.profiling) HIP_CHECK_B(hipEventRecord(b.ev[0], s));
   HIP_CHECK_B(hipGraphLaunch(b.exec, s));
@@ -383,11 +388,20 @@ void batch_set_profiling(Batch& b, bool enable) { b.profiling = enable; }
 void batch_set_launch(Batch& b, int blocks_per_cu, int chunk, int order) {
   if (blocks_per_cu < 0 || blocks_per_cu > 64) throw Error("blocks per CU must be in [0, 64] (0: one per tile)");
   if (chunk != 4 && chunk != 8) throw Error("trial chunk must be 4 or 8");
+  if (chunk == 4 && b.waves != 4) throw Error("trial chunk 4 needs 4 wavefronts per tile");
   if (order != 0 && order != 1) throw Error("tile order must be 0 (item-major) or 1 (largest items first)");
   b.blocks_per_cu = blocks_per_cu;
   b.chunk = chunk;
   b.order = order;
 }
+
+void batch_set_tile_waves(Batch& b, int waves) {
+  if (waves != 2 && waves != 4 && waves != 8) throw Error("wavefronts per tile must be 2, 4 or 8");
+  if (waves != 4 && b.chunk != 8) throw Error("2 or 8 wavefronts per tile need trial chunk 8");
+  b.waves = waves;
+}
+
+int batch_get_tile_waves(const Batch& b) { return b.waves; }
 
 void batch_get_launch(const Batch& b, int* blocks_per_cu, int* chunk, int* order) {
   *blocks_per_cu = b.blocks_per_cu;
@@ -429,7 +443,7 @@ std::string signature(const Batch& b) {
   return s.str();
 }
 
-bool cache_lookup(const char* path, const std::string& key, int* bpc, int* ch, int* ord, double* ms) {
+bool cache_lookup(const char* path, const std::string& key, int* bpc, int* ch, int* ord, int* nw, double* ms) {
   if (!path || !*path) return false;
   std::ifstream f(path);
   std::string line;
@@ -445,9 +459,13 @@ bool cache_lookup(const char* path, const std::string& key, int* bpc, int* ch, i
     const char* t = strstr(line.c_str(), "\"ms\":");
     if (p && q && r && t && sscanf(p, "\"blocks_per_cu\": %d", &a) == 1 && sscanf(q, "\"chunk\": %d", &c) == 1 &&
         sscanf(r, "\"order\": %d", &o) == 1 && sscanf(t, "\"ms\": %lf", &m) == 1) {
+      const char* w = strstr(line.c_str(), "\"waves\":");
+      int wv = 4;  // records written before the tile split was tuned
+      if (w && sscanf(w, "\"waves\": %d", &wv) != 1) wv = 4;
       *bpc = a;
       *ch = c;
       *ord = o;
+      *nw = wv;
       *ms = m;
       found = true;  // the last record of a key wins
     }
@@ -455,12 +473,12 @@ bool cache_lookup(const char* path, const std::string& key, int* bpc, int* ch, i
   return found;
 }
 
-void cache_store(const char* path, const std::string& key, int bpc, int ch, int ord, double ms) {
+void cache_store(const char* path, const std::string& key, int bpc, int ch, int ord, int nw, double ms) {
   if (!path || !*path) return;
   std::ofstream f(path, std::ios::app);
   if (!f) throw Error(std::string("cannot write the autotune cache ") + path);
   f << "{\"key\": \"" << key << "\", \"blocks_per_cu\": " << bpc << ", \"chunk\": " << ch << ", \"order\": " << ord
-    << ", \"ms\": " << ms << "}\n";
+    << ", \"waves\": " << nw << ", \"ms\": " << ms << "}\n";
 }
 
 double time_shape(Batch& b, int reps) {
@@ -493,41 +511,49 @@ double time_shape(Batch& b, int reps) {
 }  // namespace
 
 // Coordinate search over the launch shape: trial chunk, then workgroups per CU
-// (including one workgroup per tile), then tile order; each candidate is the best of
-// `reps` timed launches. The batch's outputs are overwritten. Returns the best time.
+// (including one workgroup per tile), then tile order, then the tile split (2, 4 or 8
+// wavefronts per 64 x 64 tile; the workgroup size); each candidate is the best of `reps`
+// timed launches. The batch's outputs are overwritten. Returns the best time.
 double batch_autotune(Batch& b, const char* cache_path, int reps, int* from_cache) {
   check_versions(b);
   if (reps < 1) reps = 1;
   const std::string key = signature(b);
-  int bpc, ch, ord;
+  int bpc, ch, ord, nw;
   double ms;
   *from_cache = 0;
-  if (cache_lookup(cache_path, key, &bpc, &ch, &ord, &ms)) {
-    batch_set_launch(b, bpc, ch, ord);
+  auto set_shape = [&](int p, int c, int o, int w) {
+    b.waves = 4;  // a valid intermediate state for any chunk
+    batch_set_launch(b, p, c, o);
+    batch_set_tile_waves(b, w);
+  };
+  if (cache_lookup(cache_path, key, &bpc, &ch, &ord, &nw, &ms)) {
+    set_shape(bpc, ch, ord, nw);
     *from_cache = 1;
     return ms;
   }
-  int best[3] = {2, 8, 0};
-  batch_set_launch(b, best[0], best[1], best[2]);
+  int best[4] = {2, 8, 0, 4};
+  set_shape(best[0], best[1], best[2], best[3]);
   double best_ms = time_shape(b, reps);
-  auto consider = [&](int p, int c, int o) {
-    batch_set_launch(b, p, c, o);
+  auto consider = [&](int p, int c, int o, int w) {
+    set_shape(p, c, o, w);
     const double t = time_shape(b, reps);
     if (t < best_ms) {
       best_ms = t;
       best[0] = p;
       best[1] = c;
       best[2] = o;
+      best[3] = w;
     }
   };
-  consider(best[0], 4, best[2]);
+  consider(best[0], 4, best[2], 4);
   const int bpcs[] = {1, 3, 4, 6, 8, 0};
   const int c_best = best[1];
-  for (int p : bpcs) consider(p, c_best, best[2]);
-  consider(best[0], best[1], 1 - best[2]);
-  batch_set_launch(b, best[0], best[1], best[2]);
+  for (int p : bpcs) consider(p, c_best, best[2], 4);
+  consider(best[0], best[1], 1 - best[2], 4);
+  for (int w : {2, 8}) consider(best[0], 8, best[2], w);
+  set_shape(best[0], best[1], best[2], best[3]);
   HIP_CHECK_B(hipStreamSynchronize(own_stream(b)));
-  cache_store(cache_path, key, best[0], best[1], best[2], best_ms);
+  cache_store(cache_path, key, best[0], best[1], best[2], best[3], best_ms);
   return best_ms;
 }
 

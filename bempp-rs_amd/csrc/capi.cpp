@@ -680,6 +680,27 @@ int boundary_assembler_batch_get_launch(const BoundaryAssemblerBatchT* batch, in
       1);
 }
 
+int boundary_assembler_batch_set_tile_waves(BoundaryAssemblerBatchT* batch, int waves) {
+  return guarded(
+      [&]() {
+        need(batch, "batch");
+        batch_set_tile_waves(*batch->b, waves);
+        return 0;
+      },
+      1);
+}
+
+int boundary_assembler_batch_get_tile_waves(const BoundaryAssemblerBatchT* batch, int* waves) {
+  return guarded(
+      [&]() {
+        need(batch, "batch");
+        need(waves, "waves");
+        *waves = batch_get_tile_waves(*batch->b);
+        return 0;
+      },
+      1);
+}
+
 int boundary_assembler_batch_counts(const BoundaryAssemblerBatchT* batch, int64_t* counts) {
   return guarded(
       [&]() {

@@ -197,6 +197,8 @@ void batch_set_graph(Batch& b, bool enable);
 void batch_set_profiling(Batch& b, bool enable);
 void batch_set_launch(Batch& b, int blocks_per_cu, int chunk, int order);
 void batch_get_launch(const Batch& b, int* blocks_per_cu, int* chunk, int* order);
+void batch_set_tile_waves(Batch& b, int waves);
+int batch_get_tile_waves(const Batch& b);
 void batch_counts(const Batch& b, long long* out4);
 void batch_kernel_times(const Batch& b, float* ms2);
 double batch_autotune(Batch& b, const char* cache_path, int reps, int* from_cache);

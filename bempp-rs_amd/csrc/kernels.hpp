@@ -871,13 +871,14 @@ __device__ __forceinline__ cplx p0_pair_generic(const double* tpts, int nq, cons
   return acc;
 }
 
-// CH: trial cells per chunk (a wavefront covers 16 trial cells in 16 / CH chunks);
+// NW: wavefronts per tile (each covers kTile / NW trial cells; the workgroup has 64 NW
+// threads); CH: trial cells per chunk (a wavefront covers its trial cells in chunks of CH);
 // the mirror transpose stores CH contiguous entries per segment.
 // Args: RegularArgs, possibly qualified with the constant address space (batched
 // launches read each item's arguments in place, with scalar loads).
-template <int Q, int OP, bool HELM, int CENTRED, int CH = kChunk, typename Args>
+template <int Q, int OP, bool HELM, int CENTRED, int CH = kChunk, int NW = 4, typename Args>
 __device__ __forceinline__ void p0_tile(Args& A, const int2 tile, int symmetric,
-                                        double2 (&lds)[4][CH][kTile]) {
+                                        double2 (&lds)[NW][CH][kTile]) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: lets trial loads be scalar
   const int i = tile.x * kTile + lane;
@@ -912,9 +913,11 @@ __device__ __forceinline__ void p0_tile(Args& A, const int2 tile, int symmetric,
   const auto trial_pts = constant_view(A.trial_pts);
   const auto trial_geo = constant_view(A.trial_geo);
   const auto trial_vid = constant_view(A.trial_vid);
-  static_assert(CH == 4 || CH == 8 || CH == 16, "chunk must divide 16 and 64");
-  for (int c = 0; c < 16 / CH; ++c) {
-    const int jbase = tile.y * kTile + wave * 16 + c * CH;
+  constexpr int TPW = kTile / NW;  // trial cells per wavefront
+  static_assert((NW == 2 || NW == 4 || NW == 8) && (CH == 4 || CH == 8 || CH == 16) && TPW % CH == 0,
+                "chunk must divide the trial cells of a wavefront");
+  for (int c = 0; c < TPW / CH; ++c) {
+    const int jbase = tile.y * kTile + wave * TPW + c * CH;
 #pragma unroll 1
     for (int jj = 0; jj < CH; ++jj) {
       const int j = jbase + jj;
@@ -1002,11 +1005,11 @@ struct BatchTile {
   int item, a, b, pad;
 };
 
-template <int Q, int OP, bool HELM, int CENTRED, int CH>
-__global__ void __launch_bounds__(256) batched_p0_persistent(const RegularArgs* __restrict__ items,
-                                                             const BatchTile* __restrict__ tiles, int ntiles,
-                                                             int* counter, int symmetric) {
-  __shared__ double2 lds[4][CH][kTile];
+template <int Q, int OP, bool HELM, int CENTRED, int CH, int NW = 4>
+__global__ void __launch_bounds__(64 * NW) batched_p0_persistent(const RegularArgs* __restrict__ items,
+                                                                 const BatchTile* __restrict__ tiles, int ntiles,
+                                                                 int* counter, int symmetric) {
+  __shared__ double2 lds[NW][CH][kTile];
   __shared__ int next;
   for (;;) {
     if (threadIdx.x == 0) next = atomicAdd(counter, 1);
@@ -1015,7 +1018,7 @@ __global__ void __launch_bounds__(256) batched_p0_persistent(const RegularArgs* 
     __syncthreads();  // everyone has read `next` before thread 0 overwrites it
     if (t >= ntiles) return;
     const BatchTile bt = load_uniform(tiles + t);
-    p0_tile<Q, OP, HELM, CENTRED, CH>(constant_view(items)[bt.item], make_int2(bt.a, bt.b), symmetric, lds);
+    p0_tile<Q, OP, HELM, CENTRED, CH, NW>(constant_view(items)[bt.item], make_int2(bt.a, bt.b), symmetric, lds);
   }
 }
 

@@ -166,13 +166,13 @@ __device__ __forceinline__ void far_accumulate_scaled(float sh, float dn, float 
 //   s_pq = r^2 - R^2 = a_p - b_q - 2 o_p.o_q,  a_p = o_p.(2D + o_p),  b_q = o_q.(2D - o_q),
 // so a cell pair costs 2Q short dot products and a point pair one 3-term dot product.
 // The double-layer normal products split the same way: (x - y).n = (D + o_p).n - o_q.n.
-template <int Q, int OP, bool HELM, int PH, int CH = kChunk, typename Args>
+template <int Q, int OP, bool HELM, int PH, int CH = kChunk, int NW = 4, typename Args>
 __device__ __forceinline__ void p0_tile_mixed(Args& A, const int2 tile, int symmetric,
-                                              double2 (&lds)[4][CH][kTile]) {
+                                              double2 (&lds)[NW][CH][kTile]) {
   if constexpr (HELM) {
     // k = 0: the Helmholtz kernels reduce to the Laplace ones (and the lengths cannot be scaled by k)
     if (A.k == 0.0) {
-      p0_tile_mixed<Q, OP, false, PH, CH>(A, tile, symmetric, lds);
+      p0_tile_mixed<Q, OP, false, PH, CH, NW>(A, tile, symmetric, lds);
       return;
     }
   }
@@ -214,9 +214,11 @@ __device__ __forceinline__ void p0_tile_mixed(Args& A, const int2 tile, int symm
   const float4* trial_vtx32 = (const float4*)A.trial_vtx32;
   const auto trial_geo = constant_view(A.trial_geo);
   const auto trial_vid = constant_view(A.trial_vid);
-  static_assert(CH == 4 || CH == 8 || CH == 16, "chunk must divide 16 and 64");
-  for (int c = 0; c < 16 / CH; ++c) {
-    const int jbase = tile.y * kTile + wave * 16 + c * CH;
+  constexpr int TPW = kTile / NW;  // trial cells per wavefront
+  static_assert((NW == 2 || NW == 4 || NW == 8) && (CH == 4 || CH == 8 || CH == 16) && TPW % CH == 0,
+                "chunk must divide the trial cells of a wavefront");
+  for (int c = 0; c < TPW / CH; ++c) {
+    const int jbase = tile.y * kTile + wave * TPW + c * CH;
 #pragma unroll 1
     for (int jj = 0; jj < CH; ++jj) {
       const int j = jbase + jj;
@@ -775,11 +777,11 @@ __global__ void __launch_bounds__(256) near_pair_kernel(const RegularArgs A, con
 // without spilling (SL/HS, 4-entry mirror segments), otherwise 3.
 constexpr int batched_mixed_waves(int op, int ch) { return (op == kSL || op == kHS) && ch == 4 ? 4 : 3; }
 
-template <int Q, int OP, bool HELM, int PH, int CH>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(batched_mixed_waves(OP, CH)))) batched_p0_mixed_persistent(const RegularArgs* __restrict__ items,
-                                                                   const BatchTile* __restrict__ tiles, int ntiles,
-                                                                   int* counter, int symmetric) {
-  __shared__ double2 lds[4][CH][kTile];
+template <int Q, int OP, bool HELM, int PH, int CH, int NW = 4>
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(batched_mixed_waves(OP, CH))))
+batched_p0_mixed_persistent(const RegularArgs* __restrict__ items, const BatchTile* __restrict__ tiles, int ntiles,
+                            int* counter, int symmetric) {
+  __shared__ double2 lds[NW][CH][kTile];
   __shared__ int next;
   for (;;) {
     if (threadIdx.x == 0) next = atomicAdd(counter, 1);
@@ -788,7 +790,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(batche
     __syncthreads();
     if (t >= ntiles) return;
     const BatchTile bt = load_uniform(tiles + t);
-    p0_tile_mixed<Q, OP, HELM, PH, CH>(constant_view(items)[bt.item], make_int2(bt.a, bt.b), symmetric, lds);
+    p0_tile_mixed<Q, OP, HELM, PH, CH, NW>(constant_view(items)[bt.item], make_int2(bt.a, bt.b), symmetric, lds);
   }
 }
 

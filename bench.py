@@ -78,15 +78,15 @@ def cpu_baseline(points, cells, k, target_s, threads):
     return rows * n / t, rows, t
 
 
-def batch_items(seed, n, dtype, mixed):
+def batch_items(seed, n, dtype, mixed, levels=(2, 5)):
     """The batched workload (bempp_amd.workloads.batch_spec): n ellipsoids, level uniform in
-    {2..5}, semi-axes uniform in [0.5, 2], k uniform in [1, 10]."""
+    levels (default {2..5}), semi-axes uniform in [0.5, 2], k uniform in [1, 10]."""
     from bempp_amd.workloads import batch_items as items
 
-    return items(seed, n, dtype, mixed)
+    return items(seed, n, dtype, mixed, levels=levels)
 
 
-def batch_leg(seed, n, steps, mixed):
+def batch_leg(seed, n, steps, mixed, levels=(2, 5)):
     """One persistent (graph-replayed, autotuned) launch for all items, timed, against one
     assembly launch per item; sample rows of three items checked against the oracle."""
     import numpy as np
@@ -96,7 +96,7 @@ def batch_leg(seed, n, steps, mixed):
     from bempp_amd.assembly.batch import BatchAssembler
 
     dtype = np.complex64 if mixed else np.complex128
-    items = batch_items(seed, n, dtype, mixed)
+    items = batch_items(seed, n, dtype, mixed, levels)
     entries = sum(len(g.cells) ** 2 for g, *_ in items)
     t0 = time.perf_counter()
     b = BatchAssembler([it[2] for it in items], [it[1] for it in items])
@@ -110,6 +110,19 @@ def batch_leg(seed, n, steps, mixed):
         b.assemble()
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / steps
+    tuned = b.launch()
+    # the untuned default shape, for the tuning gain
+    b.set_launch()
+    b.assemble()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.assemble()
+    torch.cuda.synchronize()
+    default_ms = (time.perf_counter() - t0) * 1e3 / steps
+    b.set_launch(*tuned)
+    b.assemble()
+    torch.cuda.synchronize()
     # sample rows of the smallest, a median and the largest item against the FP64 oracle
     order = np.argsort([len(g.cells) for g, *_ in items], kind="stable")
     errs = []
@@ -131,7 +144,8 @@ def batch_leg(seed, n, steps, mixed):
     per_item_ms = (time.perf_counter() - t0) * 1e3 / steps
     res = {"ms_per_step": ms, "value": entries / (ms * 1e-3), "unit": "entries/s",
            "per_item_launches_ms": per_item_ms, "speedup_vs_per_item_launches": per_item_ms / ms,
-           "launch_shape": dict(zip(("blocks_per_cu", "chunk", "order"), b.launch())), "autotune_best_ms": tune_ms,
+           "launch_shape": dict(zip(("blocks_per_cu", "chunk", "order", "waves"), tuned)), "autotune_best_ms": tune_ms,
+           "default_shape_ms": default_ms,
            "counts": b.counts(), "setup_s": setup_s, "sample_rel_err_vs_oracle": max(errs),
            "output": "complex64" if mixed else "complex128"}
     del b, items
@@ -470,6 +484,17 @@ def main():
             print(f"[bench] batch {'mixed c64' if bmixed else 'fp64 c128'}: {res['ms_per_step']:.2f} ms, per-item "
                   f"launches {res['per_item_launches_ms']:.2f} ms, sample err {res['sample_rel_err_vs_oracle']:.2e}",
                   file=sys.stderr, flush=True)
+        # many small meshes, where one launch sequence per item is dominated by launch overhead
+        small = {"workload": f"{args.batch_items} Helmholtz SL P0 assemblies on level-2 icospheres (320 cells) "
+                             f"scaled to ellipsoids (semi-axes uniform in [0.5, 2]), k uniform in [1, 10], "
+                             f"numpy seed {args.batch_seed}"}
+        for bmixed in (False, True):
+            e, res = batch_leg(args.batch_seed, args.batch_items, max(args.batch_steps, 20), bmixed, levels=(2, 2))
+            small["entries"] = e
+            small["mixed_complex64" if bmixed else "fp64_complex128"] = res
+            print(f"[bench] small-mesh batch {'mixed c64' if bmixed else 'fp64 c128'}: {res['ms_per_step']:.3f} ms, "
+                  f"per-item launches {res['per_item_launches_ms']:.3f} ms", file=sys.stderr, flush=True)
+        batched["small_meshes"] = small
 
     cpu = None
     if not args.no_cpu_baseline:

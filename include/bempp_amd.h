@@ -315,6 +315,10 @@ int boundary_assembler_batch_set_graph(BoundaryAssemblerBatchT* batch, int enabl
 int boundary_assembler_batch_set_launch(BoundaryAssemblerBatchT* batch, int blocks_per_cu, int chunk, int order);
 int boundary_assembler_batch_get_launch(const BoundaryAssemblerBatchT* batch, int* blocks_per_cu, int* chunk,
                                         int* order);
+/* Tile split: 2, 4 (default) or 8 wavefronts per 64 x 64 pair tile (workgroups of 64 x waves
+ * threads, each wavefront covering 64 / waves trial cells); 2 and 8 need trial chunk 8. */
+int boundary_assembler_batch_set_tile_waves(BoundaryAssemblerBatchT* batch, int waves);
+int boundary_assembler_batch_get_tile_waves(const BoundaryAssemblerBatchT* batch, int* waves);
 /* counts[4] = items, pair tiles, adjacent pairs, near pairs */
 int boundary_assembler_batch_counts(const BoundaryAssemblerBatchT* batch, int64_t* counts);
 /* Times candidate launch shapes (best of `reps` launches each; outputs are

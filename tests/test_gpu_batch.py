@@ -89,17 +89,17 @@ def test_batch_launch_shapes_agree():
     b = BatchAssembler([a for *_, a, _ in items], [s for _, s, _, _ in items])
     b.set_launch(2, 8, 0)
     ref = [o.cpu().clone() for o in b.assemble()]
-    for bpc in (0, 1, 4):
-        for chunk in (4, 8):
-            for order in (0, 1):
-                for o in b.outputs:
-                    o.fill_(float("nan"))
-                b.set_launch(bpc, chunk, order)
-                assert b.launch() == (bpc, chunk, order)
-                outs = b.assemble()
-                torch.cuda.synchronize()
-                for r, o in zip(ref, outs):
-                    assert torch.equal(r, o.cpu()), (bpc, chunk, order)
+    shapes = [(bpc, chunk, order, 4) for bpc in (0, 1, 4) for chunk in (4, 8) for order in (0, 1)]
+    shapes += [(bpc, 8, order, waves) for bpc in (0, 2) for order in (0, 1) for waves in (2, 8)]
+    for shape in shapes:
+        for o in b.outputs:
+            o.fill_(float("nan"))
+        b.set_launch(*shape)
+        assert b.launch() == shape
+        outs = b.assemble()
+        torch.cuda.synchronize()
+        for r, o in zip(ref, outs):
+            assert torch.equal(r, o.cpu()), shape
 
 
 def test_batch_graph_replays():
@@ -192,3 +192,7 @@ def test_batch_errors():
         b.assemble()
     with pytest.raises(ValueError, match="chunk"):
         b.set_launch(2, 5, 0)
+    with pytest.raises(ValueError, match="wavefronts"):
+        b.set_launch(2, 8, 0, waves=3)
+    with pytest.raises(ValueError, match="chunk 8"):
+        b.set_launch(2, 4, 0, waves=8)
