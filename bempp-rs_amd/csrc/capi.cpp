@@ -62,38 +62,6 @@ void need(const void* p, const char* what) {
   if (!p) throw Error(std::string(what) + " is null");
 }
 
-// COO -> CSR with duplicates summed, rows then columns sorted (rlst
-// CsrMatrix::from_aij semantics, src/boundary_assemblers.rs:147-153); empty
-// when every |entry| <= 1e-10 (the hack at :130-145).
-template <typename T>
-void build_csr(MatrixT* m, const std::vector<int64_t>& rows, const std::vector<int64_t>& cols, const T* vals) {
-  const size_t n = rows.size();
-  m->indptr.assign(m->shape0 + 1, 0);
-  bool any = false;
-  for (size_t i = 0; i < n; ++i)
-    if (std::abs(vals[i]) > 1e-10) { any = true; break; }
-  if (n == 0 || !any) return;
-  std::vector<size_t> order(n);
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
-    return rows[a] != rows[b] ? rows[a] < rows[b] : cols[a] < cols[b];
-  });
-  std::vector<T> out;
-  for (size_t k = 0; k < n; ++k) {
-    const size_t i = order[k];
-    if (!m->indices.empty() && k > 0 && rows[order[k - 1]] == rows[i] && cols[order[k - 1]] == cols[i]) {
-      out.back() += vals[i];
-    } else {
-      m->indices.push_back((uintptr_t)cols[i]);
-      out.push_back(vals[i]);
-      m->indptr[rows[i] + 1]++;
-    }
-  }
-  for (size_t r = 0; r < m->shape0; ++r) m->indptr[r + 1] += m->indptr[r];
-  m->data.resize(out.size() * sizeof(T));
-  std::memcpy(m->data.data(), out.data(), m->data.size());
-}
-
 }  // namespace
 
 extern "C" {
@@ -464,29 +432,18 @@ MatrixT* boundary_assembler_assemble(const BoundaryAssemblerT* a, const SpaceT* 
 
 namespace {
 
-MatrixT* coo_to_matrix(const BoundaryAssemblerT* a, const SpaceT* trial, const SpaceT* test,
-                       const std::vector<int64_t>& rows, const std::vector<int64_t>& cols,
-                       const std::vector<double>& vals) {
+// The device-built CSR (src/boundary_assemblers.rs:122-155) as a MatrixT.
+MatrixT* singular_csr_matrix(const BoundaryAssemblerT* a, const SpaceT* trial, const SpaceT* test, bool correction) {
+  CsrHost c;
+  assemble_singular_csr(*a->a, *trial->s, *test->s, correction, c);
   auto* m = new MatrixT();
   m->csr = true;
   m->dtype = a->a->dtype;
   m->shape0 = (size_t)test->s->size;
   m->shape1 = (size_t)trial->s->size;
-  switch (m->dtype) {
-    case kF64: build_csr<double>(m, rows, cols, vals.data()); break;
-    case kC64: build_csr<std::complex<double>>(m, rows, cols, (const std::complex<double>*)vals.data()); break;
-    case kF32: {
-      std::vector<float> v(vals.begin(), vals.end());
-      build_csr<float>(m, rows, cols, v.data());
-      break;
-    }
-    default: {
-      std::vector<std::complex<float>> v(vals.size() / 2);
-      for (size_t i = 0; i < v.size(); ++i) v[i] = {(float)vals[2 * i], (float)vals[2 * i + 1]};
-      build_csr<std::complex<float>>(m, rows, cols, v.data());
-      break;
-    }
-  }
+  m->indptr.assign(c.indptr.begin(), c.indptr.end());
+  m->indices.assign(c.indices.begin(), c.indices.end());
+  m->data = std::move(c.data);
   return m;
 }
 
@@ -496,10 +453,7 @@ MatrixT* boundary_assembler_assemble_singular(const BoundaryAssemblerT* a, const
   return guarded(
       [&]() -> MatrixT* {
         check_spaces(a, trial, test);
-        std::vector<int64_t> rows, cols;
-        std::vector<double> vals;
-        assemble_singular_coo(*a->a, *trial->s, *test->s, rows, cols, vals);
-        return coo_to_matrix(a, trial, test, rows, cols, vals);
+        return singular_csr_matrix(a, trial, test, false);
       },
       (MatrixT*)nullptr);
 }
@@ -533,10 +487,7 @@ MatrixT* boundary_assembler_assemble_singular_correction(const BoundaryAssembler
   return guarded(
       [&]() -> MatrixT* {
         check_spaces(a, trial, test);
-        std::vector<int64_t> rows, cols;
-        std::vector<double> vals;
-        assemble_singular_correction_coo(*a->a, *trial->s, *test->s, rows, cols, vals);
-        return coo_to_matrix(a, trial, test, rows, cols, vals);
+        return singular_csr_matrix(a, trial, test, true);
       },
       (MatrixT*)nullptr);
 }

@@ -151,6 +151,17 @@ void assemble_singular_coo(const Assembler& a, const FunctionSpace& trial, const
 void assemble_singular_correction_coo(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test,
                                       std::vector<int64_t>& rows, std::vector<int64_t>& cols,
                                       std::vector<double>& vals);
+// CSR matrix in host memory: row pointers [nrows + 1], column indices and values (in the
+// assembler dtype, interleaved for complex) of the nnz stored entries.
+struct CsrHost {
+  std::vector<int64_t> indptr, indices;
+  std::vector<unsigned char> data;
+};
+// The singular part (correction = false) or the singular correction (true) as CSR, built on
+// the device: columns ascending within each row, duplicate entries summed in COO order, and
+// no stored entries when no |entry| exceeds 1e-10.
+void assemble_singular_csr(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test, bool correction,
+                           CsrHost& out);
 // Potential operator [npoints x ndofs] (column-major) of a single/double-layer assembler
 // on `space` at the given points; device variant (points and output in device memory,
 // on `stream`) and host variant (accumulates into a host buffer).

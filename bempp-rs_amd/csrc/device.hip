@@ -1040,52 +1040,6 @@ void assemble_host(const Assembler& a, const FunctionSpace& trial, const Functio
   HIP_CHECK(hipFree(d));
 }
 
-void assemble_singular_coo(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test,
-                           std::vector<int64_t>& rows, std::vector<int64_t>& cols, std::vector<double>& vals) {
-  rows.clear();
-  cols.clear();
-  vals.clear();
-  if (test.grid != trial.grid) return;  // no neighbouring cells across grids (:240-243)
-  Prepared P = prepare(a, trial, test, nullptr, nullptr);
-  if (P.npairs == 0) return;
-  const int nbl = test.nb * trial.nb;
-  const size_t nvals = (size_t)P.npairs * nbl;
-  double* d = nullptr;
-  HIP_CHECK(hipMalloc((void**)&d, nvals * 2 * sizeof(double)));
-  std::vector<double> h(nvals * 2);
-  try {
-    dev::SingularArgs S = P.S;
-    S.mode = dev::kCoo;
-    S.out = d;
-    dim3 sg((unsigned)((P.npairs + 3) / 4));
-    launch_singular(a.op, a.kind == kHelmholtz, test.nb == 1 && trial.nb == 1, P.small_adjacent, sg, P.stream, S);
-    HIP_CHECK(hipMemcpyAsync(h.data(), d, h.size() * sizeof(double), hipMemcpyDeviceToHost, P.stream));
-    HIP_CHECK(hipStreamSynchronize(P.stream));
-  } catch (...) {
-    (void)hipFree(d);
-    throw;
-  }
-  HIP_CHECK(hipFree(d));
-  const auto& pairs = const_cast<Grid&>(*test.grid).device_cache.at(P.device)->host_pairs;
-  rows.resize(nvals);
-  cols.resize(nvals);
-  const bool cplx = is_complex(a.dtype);
-  vals.resize(nvals * (cplx ? 2 : 1));
-  for (size_t p = 0; p < pairs.size(); ++p)
-    for (int b = 0; b < trial.nb; ++b)
-      for (int t = 0; t < test.nb; ++t) {
-        const size_t e = p * nbl + t + test.nb * b;
-        rows[e] = test.dofs[(size_t)pairs[p].test * test.nb + t];
-        cols[e] = trial.dofs[(size_t)pairs[p].trial * trial.nb + b];
-        if (cplx) {
-          vals[2 * e] = h[2 * e];
-          vals[2 * e + 1] = h[2 * e + 1];
-        } else {
-          vals[e] = h[2 * e];
-        }
-      }
-}
-
 template <int OP, bool HELM>
 void launch_correction_q(int Q, dim3 grid, hipStream_t s, const dev::RegularArgs& R, const int4* pairs, int n,
                          double* out) {
@@ -1110,27 +1064,48 @@ void launch_correction(int op, int Q, dim3 grid, hipStream_t s, const dev::Regul
   }
 }
 
-// Regular-rule contributions of the adjacent pairs as COO triplets (same layout as
-// assemble_singular_coo, cells' own local basis order).
-void assemble_singular_correction_coo(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test,
-                                      std::vector<int64_t>& rows, std::vector<int64_t>& cols,
-                                      std::vector<double>& vals) {
+// Values of the adjacent pairs in COO layout [npairs][nbt * nbs] (re, im), position
+// pair * nbt * nbs + t + nbt * b: the Duffy rules (correction = false) or the regular rule
+// (the singular correction). Returns a new device buffer; the caller frees it.
+static double* adjacent_coo_values(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test,
+                                   const Prepared& P, bool correction) {
+  const size_t nvals = (size_t)P.npairs * test.nb * trial.nb;
+  double* d = nullptr;
+  HIP_CHECK(hipMalloc((void**)&d, nvals * 2 * sizeof(double)));
+  try {
+    if (correction) {
+      const dim3 grid((unsigned)((P.npairs + 255) / 256));
+      if (a.kind == kHelmholtz) launch_correction<true>(a.op, P.Q, grid, P.stream, P.R, P.S.pairs, P.npairs, d);
+      else launch_correction<false>(a.op, P.Q, grid, P.stream, P.R, P.S.pairs, P.npairs, d);
+    } else {
+      dev::SingularArgs S = P.S;
+      S.mode = dev::kCoo;
+      S.out = d;
+      dim3 sg((unsigned)((P.npairs + 3) / 4));
+      launch_singular(a.op, a.kind == kHelmholtz, test.nb == 1 && trial.nb == 1, P.small_adjacent, sg, P.stream, S);
+    }
+    HIP_CHECK(hipGetLastError());
+  } catch (...) {
+    (void)hipFree(d);
+    throw;
+  }
+  return d;
+}
+
+// COO triplets on the host (cells' own local basis order, the adjacent-pair order).
+static void adjacent_coo(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test, bool correction,
+                         std::vector<int64_t>& rows, std::vector<int64_t>& cols, std::vector<double>& vals) {
   rows.clear();
   cols.clear();
   vals.clear();
-  if (test.grid != trial.grid) return;  // no adjacent pairs across grids
+  if (test.grid != trial.grid) return;  // no neighbouring cells across grids (:240-243)
   Prepared P = prepare(a, trial, test, nullptr, nullptr);
   if (P.npairs == 0) return;
   const int nbl = test.nb * trial.nb;
   const size_t nvals = (size_t)P.npairs * nbl;
-  double* d = nullptr;
-  HIP_CHECK(hipMalloc((void**)&d, nvals * 2 * sizeof(double)));
   std::vector<double> h(nvals * 2);
+  double* d = adjacent_coo_values(a, trial, test, P, correction);
   try {
-    const dim3 grid((unsigned)((P.npairs + 255) / 256));
-    if (a.kind == kHelmholtz) launch_correction<true>(a.op, P.Q, grid, P.stream, P.R, P.S.pairs, P.npairs, d);
-    else launch_correction<false>(a.op, P.Q, grid, P.stream, P.R, P.S.pairs, P.npairs, d);
-    HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(h.data(), d, h.size() * sizeof(double), hipMemcpyDeviceToHost, P.stream));
     HIP_CHECK(hipStreamSynchronize(P.stream));
   } catch (...) {
@@ -1156,6 +1131,40 @@ void assemble_singular_correction_coo(const Assembler& a, const FunctionSpace& t
           vals[e] = h[2 * e];
         }
       }
+}
+
+void assemble_singular_coo(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test,
+                           std::vector<int64_t>& rows, std::vector<int64_t>& cols, std::vector<double>& vals) {
+  adjacent_coo(a, trial, test, false, rows, cols, vals);
+}
+
+void assemble_singular_correction_coo(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test,
+                                      std::vector<int64_t>& rows, std::vector<int64_t>& cols,
+                                      std::vector<double>& vals) {
+  adjacent_coo(a, trial, test, true, rows, cols, vals);
+}
+
+void coo_to_csr_device(const double* vals, const int4* pairs, int npairs, const int* test_dofs, int nbt,
+                       const int* trial_dofs, int nbs, int64_t nrows, int64_t ncols, DType dtype, hipStream_t s,
+                       CsrHost& out);  // csr.hip
+
+void assemble_singular_csr(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test, bool correction,
+                           CsrHost& out) {
+  out.indptr.assign((size_t)test.size + 1, 0);
+  out.indices.clear();
+  out.data.clear();
+  if (test.grid != trial.grid) return;  // no neighbouring cells across grids (:240-243)
+  Prepared P = prepare(a, trial, test, nullptr, nullptr);
+  if (P.npairs == 0) return;
+  double* d = adjacent_coo_values(a, trial, test, P, correction);
+  try {
+    coo_to_csr_device(d, P.S.pairs, P.npairs, P.S.test_dofs, test.nb, P.S.trial_dofs, trial.nb, test.size, trial.size,
+                      a.dtype, P.stream, out);
+  } catch (...) {
+    (void)hipFree(d);
+    throw;
+  }
+  HIP_CHECK(hipFree(d));
 }
 
 // ---- potential (off-surface) operators ----

@@ -200,6 +200,40 @@ def shape_gradient_leg(level, warmup, steps):
             "cell_pairs_per_s": n * n / (ms * 1e-3), "rel_diff_vs_allocating_entry_point": diff}
 
 
+def singular_csr_leg(level, k, steps):
+    """assemble_singular (the adjacent-pair part as a CSR matrix in host memory: Duffy kernel in
+    COO mode, then keys, radix sort, duplicate sums and row pointers on the device) on the
+    level-`level` icosphere: Helmholtz SL P0 (no duplicates) and Laplace HS P1 (duplicates)."""
+    import numpy as np
+
+    from bempp_amd.assembly.boundary import (BoundaryAssemblerOptions, OperatorType, create_helmholtz_assembler,
+                                             create_laplace_assembler)
+    from bempp_amd.element import Continuity, Family, ReferenceCellType, create_family
+    from bempp_amd.function_space import function_space
+    from bempp_amd.shapes import icosphere
+
+    g = icosphere(level)
+    o = BoundaryAssemblerOptions()
+    o.set_regular_quadrature_degree(ReferenceCellType.Triangle, 6)
+    o.set_singular_quadrature_degree(ReferenceCellType.Triangle, ReferenceCellType.Triangle, 4)
+    out = {}
+    for name, a, fam in [("helmholtz_sl_dp0", create_helmholtz_assembler(k, OperatorType.SingleLayer, o),
+                          create_family(Family.Lagrange, 0, Continuity.Discontinuous)),
+                         ("laplace_hs_p1", create_laplace_assembler(OperatorType.Hypersingular, o),
+                          create_family(Family.Lagrange, 1, Continuity.Standard))]:
+        s = function_space(g, fam)
+        a.assemble_singular(s, s)  # builds the grid's and space's device data
+        times = []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            m = a.assemble_singular(s, s)
+            times.append((time.perf_counter() - t0) * 1e3)
+        out[name] = {"ms_per_call": float(np.median(times)), "nnz": int(m.nnz), "shape": list(m.shape)}
+        print(f"[bench] singular CSR {name} L{level}: {np.median(times):.2f} ms, nnz {m.nnz}", file=sys.stderr,
+              flush=True)
+    return out
+
+
 def copy_calibration():
     """A plain device copy of 1 GiB (read 1 GiB + write 1 GiB per dispatch), three times:
     the reference point that scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -448,6 +482,7 @@ def main():
     # radially by 0.05 sum a_lm Y_lm (l <= 4, seed 0), u, v complex normal: matrix-free, device
     # inputs through the entry point that allocates nothing
     gradient = shape_gradient_leg(args.level, args.warmup, args.steps)
+    singular_csr = singular_csr_leg(args.level, args.k, max(3, args.steps))
 
     # the oracle spot check of the sample rows (first 64 rows: test cells 0..63 against all trial cells)
     ref = O.assemble("helmholtz", "single_layer", grid.points, grid.cells, k=args.k, fast=False, rows=(0, rows))
@@ -532,6 +567,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "shape_gradient": gradient,
+        "singular_csr": singular_csr,
     }
     print(json.dumps(line))
 

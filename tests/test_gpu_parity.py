@@ -147,6 +147,46 @@ def test_singular_csr_matches_oracle():
         assert np.all(np.diff(csr.indptr) >= 0) and csr.has_sorted_indices
 
 
+@pytest.mark.parametrize("level,kind,op,sp,k,dt,tol", [
+    (1, "laplace", "single_layer", "dp0", 0.0, np.float32, 2e-6),
+    (1, "helmholtz", "hypersingular", "p1", 2.0, np.complex64, 2e-6),
+    (2, "laplace", "double_layer", "p1", 0.0, np.float32, 2e-6),
+    (3, "helmholtz", "adjoint_double_layer", "p1", 1.5, np.complex128, TOL),
+    (3, "laplace", "hypersingular", "dp1", 0.0, np.float64, TOL),
+])
+def test_singular_csr_structure(level, kind, op, sp, k, dt, tol):
+    """The device-built CSR (src/boundary_assemblers.rs:122-155): its pattern is exactly the
+    distinct (row, column) pairs of the oracle's COO (duplicates summed, none dropped), columns
+    strictly ascending within each row, row pointers from 0 to nnz, the same bytes on a second
+    call, values in the assembler dtype equal to the oracle's COO summed."""
+    g = icosphere(level)
+    p, c = g.points, g.cells
+    o = _options()
+    a = (create_laplace_assembler(OPS[op], o, dtype=dt) if kind == "laplace"
+         else create_helmholtz_assembler(k, OPS[op], o, dtype=dt))
+    s = _space(g, sp)
+    print(f"[csr] level {level} {kind} {op} {sp} {np.dtype(dt).name}", flush=True)
+    csr = a.assemble_singular(s, s)
+    again = a.assemble_singular(s, s)
+    assert csr.dtype == np.dtype(dt)
+    assert np.array_equal(csr.indptr, again.indptr) and np.array_equal(csr.indices, again.indices)
+    assert np.array_equal(csr.data.view(np.uint8), again.data.view(np.uint8))
+    n = s.global_size
+    assert csr.shape == (n, n) and csr.indptr[0] == 0 and csr.indptr[-1] == csr.nnz
+    assert np.all(np.diff(csr.indptr) >= 0)
+    row_of = np.repeat(np.arange(n), np.diff(csr.indptr))
+    same_row = np.diff(row_of) == 0
+    assert np.all(np.diff(csr.indices)[same_row] > 0)
+    rows, cols, vals = O.assemble_singular_coo(kind, op, p, c, test=SPACES[sp], trial=SPACES[sp], k=k)
+    assert len(rows) > csr.nnz or sp != "p1"  # continuous spaces: duplicates were summed
+    assert np.array_equal(np.unique(rows * n + cols), row_of * n + csr.indices)
+    ref = np.zeros((n, n), dtype=vals.dtype)
+    np.add.at(ref, (rows, cols), vals)
+    err = _rel(csr.toarray().astype(ref.dtype), ref)
+    print(f"[csr]   nnz {csr.nnz} of {len(rows)} COO entries, rel err {err:.2e}", flush=True)
+    assert err < tol
+
+
 def test_separate_grids_have_no_singular_part():
     """Different grids: no neighbour skipping and no Duffy pairs (src/boundary_assemblers.rs:240-243,794-821)."""
     g1, g2 = icosphere(1), icosphere(1)
