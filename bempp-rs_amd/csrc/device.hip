@@ -806,8 +806,13 @@ Prepared prepare(const Assembler& a, const FunctionSpace& trial_c, const Functio
   R.test_vtx64 = R.orbit_form ? grid_vertex64_for_rule(tgrid, tc, P.Q, w) : nullptr;
   R.trial_vtx64 = R.orbit_form ? grid_vertex64_for_rule(sgrid, sc, P.Q, w) : nullptr;
   // mixed precision: one grid (P0 x P0: p0 mixed tiles; a P1 side: LDS mixed tiles), rules
-  // compiled for their point count; other rules assemble in FP64
-  P.mixed = a.options.precision == kMixed && P.same_grid && compiled_q;
+  // compiled for their point count; other rules assemble in FP64. The Laplace hypersingular
+  // operator on P1 x P1 also stays in FP64: its cell-pair sum is the single-layer sum, which
+  // the FP64 tiles take in orbit form, and with the P1 scatter around it that runs faster
+  // than the FP32 point pairs of the mixed tiles (FP32 without packed math issues at the
+  // FP64 rate)
+  const bool laplace_hs_p1 = a.kind == kLaplace && a.op == kHS && test.nb == 3 && trial.nb == 3 && R.orbit_form;
+  P.mixed = a.options.precision == kMixed && P.same_grid && compiled_q && !laplace_hs_p1;
   P.near = nullptr;
   P.nnear = 0;
   P.near_segs = nullptr;

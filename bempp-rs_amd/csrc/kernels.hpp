@@ -244,9 +244,20 @@ __device__ __forceinline__ void store_value(void* out, long long idx, cplx v, in
   }
 }
 
+// The hypersingular curl-curl entries are (curl_a . curl_b) msum / (J_x J_y): callers scale
+// msum by 1 / (J_x J_y) once per cell pair with this, before the entries (local_entry).
+template <int OP, typename TG, typename SG>
+__device__ __forceinline__ cplx hs_msum(cplx msum, TG tgeo, SG sgeo) {
+  if constexpr (OP == 3) {
+    const double s = 1.0 / (tgeo[12] * sgeo[12]);
+    return {msum.re * s, msum.im * s};
+  }
+  return msum;
+}
+
 // Epilogue shared by both loops: M (nbt x nbs local sums without 1/(4pi)) ->
 // local matrix entries. tgeo/sgeo: geometry records; lt/ls: local basis index
-// of each row/column of M.
+// of each row/column of M; msum: the sum of M scaled by hs_msum.
 template <int OP, bool HELM, typename TG, typename SG>
 __device__ __forceinline__ cplx local_entry(const cplx M[3][3], int a, int b, cplx msum, int nbt, int nbs,
                                             TG tgeo, SG sgeo, int la, int lb, double k) {
@@ -256,7 +267,7 @@ __device__ __forceinline__ cplx local_entry(const cplx M[3][3], int a, int b, cp
     if (nbt == 3 && nbs == 3) {
       const auto ct = tgeo + 13 + 3 * la;
       const auto cs = sgeo + 13 + 3 * lb;
-      const double cc = (ct[0] * cs[0] + ct[1] * cs[1] + ct[2] * cs[2]) / (tgeo[12] * sgeo[12]);
+      const double cc = ct[0] * cs[0] + ct[1] * cs[1] + ct[2] * cs[2];
       out = {cc * msum.re, cc * msum.im};
     }
     if constexpr (HELM) {
@@ -520,9 +531,14 @@ __global__ void __launch_bounds__(256) regular_kernel(const RegularArgs A) {
       p1_local_generic<OP, HELM, CENTRED>(A, tpts, nx, ct, spp, sg, M, msum);
     else
       p1_local<Q, OP, HELM, CENTRED>(A, xp, wp, nx, ct, sp, sg, M, msum);
-    for (int b = 0; b < A.nbs; ++b) {
+    msum = hs_msum<OP>(msum, tgeo, sg);
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      if (b >= A.nbs) continue;  // uniform; constant indices keep M in registers
       const long long col = A.trial_dofs[(size_t)j * A.nbs + b];
-      for (int a = 0; a < A.nbt; ++a) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        if (a >= A.nbt) continue;
         const long long row = A.test_dofs[(size_t)i * A.nbt + a];
         const cplx v = local_entry<OP, HELM>(M, a, b, msum, A.nbt, A.nbs, tgeo, sg, a, b, A.k);
         store_value(A.out, row + A.ld * col, v, A.out_type, kAtomic);
@@ -729,8 +745,12 @@ __global__ void __launch_bounds__(256) regular_tiled_lds(const RegularArgs A, co
   constexpr bool GEN = Q == 0;  // rule of any point count, points streamed
   constexpr int QR = GEN ? 1 : Q;
   const double* tpts = A.test_pts + (size_t)il * (GEN ? A.nq : Q) * 4;
+  // Laplace hypersingular: the P1 surface curls are constant on a cell, so a cell pair needs
+  // only sum_pq (w_p J_x)(w_q J_y) G_pq -- the single-layer P0 sum, taken in orbit form as
+  // the P0 tiles take it (p0_pair_orbit) when the rule has that form
+  constexpr bool LHS = OP == 3 && !HELM && !GEN;
   double xp[QR][3], wp[QR];
-  if constexpr (!HELM && !GEN) {
+  if constexpr (!HELM && !GEN && !LHS) {
 #pragma unroll
     for (int p = 0; p < Q; ++p) {
       xp[p][0] = tpts[4 * p];
@@ -743,10 +763,6 @@ __global__ void __launch_bounds__(256) regular_tiled_lds(const RegularArgs A, co
   const double nx[3] = {tgeo[9], tgeo[10], tgeo[11]};
   const double ct[3] = {tgeo[kGeoCentroid], tgeo[kGeoCentroid + 1], tgeo[kGeoCentroid + 2]};
   const int tv0 = A.test_vid[4 * il], tv1 = A.test_vid[4 * il + 1], tv2 = A.test_vid[4 * il + 2];
-  // Laplace hypersingular: the P1 surface curls are constant on a cell, so a cell pair needs
-  // only sum_pq (w_p J_x)(w_q J_y) G_pq -- the single-layer P0 sum, taken in orbit form as
-  // the P0 tiles take it (p0_pair_orbit) when the rule has that form
-  constexpr bool LHS = OP == 3 && !HELM && !GEN;
   const bool lhs_orbit = LHS && A.orbit_form;
   double td[2][3], tau2g[QR], twc[3];
   if constexpr (LHS) {
@@ -779,7 +795,7 @@ __global__ void __launch_bounds__(256) regular_tiled_lds(const RegularArgs A, co
     if (j >= A.n_trial) break;  // wave-uniform
     const auto spp = trial_pts + (size_t)j * (GEN ? A.nq : Q) * 4;
     double sp[QR * 4];
-    if constexpr (!GEN) {
+    if constexpr (!GEN && !LHS) {  // Laplace HS reads the trial points only off the orbit form
 #pragma unroll
       for (int m = 0; m < Q * 4; ++m) sp[m] = spp[m];
     }
@@ -810,11 +826,16 @@ __global__ void __launch_bounds__(256) regular_tiled_lds(const RegularArgs A, co
       }
     } else if constexpr (GEN) {
       p1_local_generic<OP, HELM, CENTRED>(A, tpts, nx, ct, spp, sg, M, msum);
-    } else if constexpr (HELM) {
+    } else if constexpr (HELM || LHS) {  // LHS here: a rule off the orbit form
+      if constexpr (LHS) {
+#pragma unroll
+        for (int m = 0; m < Q * 4; ++m) sp[m] = spp[m];
+      }
       p1_local_streamed<Q, OP, HELM, CENTRED>(A, tpts, nx, ct, sp, sg, M, msum);
     } else {
       p1_local<Q, OP, HELM, CENTRED>(A, xp, wp, nx, ct, sp, sg, M, msum);
     }
+    msum = hs_msum<OP>(msum, tgeo, sg);
 #pragma unroll
     for (int b = 0; b < 3; ++b)
 #pragma unroll
@@ -1137,10 +1158,17 @@ __device__ __forceinline__ void singular_pair(Args& A, const int pair, const int
       msum.im += Ml[a][b].im;
     }
   if (lane != 0) return;
+  msum = hs_msum<OP>(msum, tg, sg);
   const int nbt = P0P0 ? 1 : A.nbt, nbs = P0P0 ? 1 : A.nbs;
-  for (int b = 0; b < nbs; ++b) {
+  // loops unrolled over the largest block with uniform skips: constant indices keep Ml in
+  // registers (a runtime bound would index it dynamically, through per-lane stack memory)
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    if (b >= nbs) continue;
     const int lb = nbs == 3 ? perm_digit(code >> 8, b) : 0;
-    for (int a = 0; a < nbt; ++a) {
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+      if (a >= nbt) continue;
       const int la = nbt == 3 ? perm_digit(code, a) : 0;
       const cplx v = local_entry<OP, HELM>(Ml, a, b, msum, nbt, nbs, A.test_geo + (size_t)tc * kGeoStride,
                                            A.trial_geo + (size_t)sc * kGeoStride, la, lb, A.k);
@@ -1240,6 +1268,7 @@ __global__ void __launch_bounds__(256) correction_kernel(const RegularArgs A, co
       msum.re += M[a][b].re;
       msum.im += M[a][b].im;
     }
+  msum = hs_msum<OP>(msum, tgeo, sgeo);
   const int nbl = A.nbt * A.nbs;
 #pragma unroll
   for (int b = 0; b < 3; ++b)

@@ -577,6 +577,7 @@ __device__ __forceinline__ void p1_tile_mixed(const RegularArgs& A, const int2 t
         msum.im += M64[a][b].im;
       }
     if constexpr (kSum) msum = {(double)S * scale, 0.0};
+    msum = hs_msum<OP>(msum, tgeo, sg);
 #pragma unroll
     for (int b = 0; b < 3; ++b)
 #pragma unroll
@@ -705,6 +706,7 @@ __global__ void __launch_bounds__(256) near_pair_p1_tiled(const RegularArgs A, c
     const double ct[3] = {tg[kGeoCentroid], tg[kGeoCentroid + 1], tg[kGeoCentroid + 2]};
     cplx M[3][3], msum;
     p1_local<Q, OP, HELM, CENTRED>(A, xp, wp, nx, ct, sp, sg, M, msum);
+    msum = hs_msum<OP>(msum, tg, sg);
 #pragma unroll
     for (int b = 0; b < 3; ++b)
 #pragma unroll

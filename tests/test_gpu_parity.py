@@ -336,6 +336,9 @@ def test_mixed_precision_p1(kind, k, op):
     R = O.assemble(kind, op, g.points, g.cells, test=(1, True), trial=(1, True), k=k)
     err = _rel(A, R)
     print(f"[mixed-p1] {kind} {op} k={k}: rel Frobenius err {err:.3e}", flush=True)
+    if kind == "laplace" and op == "hypersingular":
+        assert err <= 1e-12  # assembled by the FP64 orbit-form tiles, faster than FP32 here
+        return
     assert err <= 1e-6
     assert err > 1e-13  # the FP32 path ran
 
