@@ -34,6 +34,7 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <complex>
 #include <cstdint>
@@ -981,6 +982,399 @@ int oracle_duffy_rule(int m, const int* test_local, const int* trial_local, int 
   } catch (const std::exception& e) {
     g_err = e.what();
     return -1;
+  }
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Grids with triangle and quadrilateral cells (the single-element grids of either
+// type and the mixed grids of ndgrid, degree-1 geometry), restated plainly: a loop over
+// all cell pairs with the geometry evaluated at every point. The reference's driver
+// is the same for any cell type (src/boundary_assemblers.rs:396-521 loops over the
+// grid's entity_types(2); :524-562 picks a Duffy rule per (test type, trial type)).
+//
+// Quadrilateral (ndgrid / DefElement ordering): reference vertices (0,0), (1,0),
+// (0,1), (1,1); x(xi, eta) = V0 + xi (V1 - V0) + eta (V2 - V0) + xi eta (V3 - V1 - V2 + V0);
+// P1 (Q1) basis (1-xi)(1-eta), xi(1-eta), (1-xi)eta, xi eta. Regular rule: m x m
+// Gauss-Legendre (npoints = m^2), ndelement's quadrilateral rule.
+//
+// Singular pairs: bempp-quadrature's quadrilateral Duffy rules are not available, so
+// each quadrilateral is split into two reference triangles chosen so that every
+// (test sub-triangle, trial sub-triangle) pair shares the pair's common vertex or edge,
+// and each sub-pair takes the triangle Sauter-Schwab rule above through the composite
+// map (reference triangle -> sub-triangle of the reference square -> bilinear cell):
+//   shared vertex s:      split along the diagonal through s;
+//   shared edge (s1, s2): split along the diagonal through s1, giving (s1, s2, o1) and
+//                         (s1, o1, t) (o1 opposite s1): every sub-pair shares s1, the two
+//                         sub-triangles on the edge share it;
+//   coincident:           split along V0-V3 (two coincident and two edge-adjacent
+//                         sub-pairs).
+// The rules integrate the same singular integrals as bempp-quadrature's; their point
+// sets differ, so parity with the reference's quadrilateral entries is unpinned beyond
+// the closed forms, identities and triangle agreement the tests check.
+// ===========================================================================
+namespace {
+
+struct CGrid {
+  int64_t nv = 0, nc = 0;
+  const double* pts = nullptr;
+  std::vector<int> nvc;          // vertices per cell: 3 triangle, 4 quadrilateral
+  std::vector<int64_t> off;      // first vertex of each cell in verts
+  std::vector<int64_t> verts;
+  std::vector<std::vector<int64_t>> vertex_cells;
+  CGrid(const double* p, int64_t nv_, const int64_t* cell_verts, const uint8_t* types, int64_t nc_)
+      : nv(nv_), nc(nc_), pts(p) {
+    nvc.resize(nc);
+    off.resize(nc + 1, 0);
+    for (int64_t c = 0; c < nc; ++c) {
+      if (types[c] != 2 && types[c] != 3) throw std::runtime_error("cell types must be 2 (triangle) or 3 (quadrilateral)");
+      nvc[c] = types[c] == 2 ? 3 : 4;
+      off[c + 1] = off[c] + nvc[c];
+    }
+    verts.assign(cell_verts, cell_verts + off[nc]);
+    vertex_cells.resize(nv);
+    for (int64_t c = 0; c < nc; ++c)
+      for (int m = 0; m < nvc[c]; ++m) {
+        const int64_t v = verts[off[c] + m];
+        if (v < 0 || v >= nv) throw std::runtime_error("cell vertex out of range");
+        auto& l = vertex_cells[v];
+        if (l.empty() || l.back() != c) l.push_back(c);
+      }
+  }
+  int64_t v(int64_t c, int m) const { return verts[off[c] + m]; }
+};
+
+// src/function.rs:365-445: P0 one DOF per cell; continuous P1 on vertices, numbered in
+// order of first appearance over the cells; discontinuous P1 cell by cell.
+struct CSpace {
+  const CGrid* g;
+  int degree;
+  bool continuous;
+  std::vector<int64_t> off, dofs;  // per cell: dofs[off[c] .. off[c+1])
+  int64_t size = 0;
+  CSpace(const CGrid* grid, int deg, bool cont) : g(grid), degree(deg), continuous(cont && deg > 0) {
+    if (deg != 0 && deg != 1) throw std::runtime_error("only degree 0 and 1 are supported");
+    off.resize(g->nc + 1, 0);
+    for (int64_t c = 0; c < g->nc; ++c) off[c + 1] = off[c] + (deg == 0 ? 1 : g->nvc[c]);
+    dofs.resize(off[g->nc]);
+    std::vector<int64_t> vdof(g->nv, -1);
+    for (int64_t c = 0; c < g->nc; ++c)
+      for (int64_t i = off[c]; i < off[c + 1]; ++i) {
+        if (continuous) {
+          const int64_t vv = g->v(c, (int)(i - off[c]));
+          if (vdof[vv] < 0) vdof[vv] = size++;
+          dofs[i] = vdof[vv];
+        } else {
+          dofs[i] = size++;
+        }
+      }
+  }
+  int nb(int64_t c) const { return (int)(off[c + 1] - off[c]); }
+};
+
+// Geometry and basis of cell c at reference point (x, y): point, unit normal, Jacobian
+// determinant, and per basis function its value and J-scaled surface curl
+// t_eta d_xi(phi) - t_xi d_eta(phi) (the curl times J).
+struct CPoint {
+  double x[3], n[3], jdet;
+  double phi[4], curl[4][3];
+};
+
+CPoint cell_point(const CGrid& g, int64_t c, int degree, double x, double y) {
+  CPoint P;
+  const int nvv = g.nvc[c];
+  double V[4][3];
+  for (int m = 0; m < nvv; ++m)
+    for (int d = 0; d < 3; ++d) V[m][d] = g.pts[3 * g.v(c, m) + d];
+  double tx[3], ty[3];
+  for (int d = 0; d < 3; ++d) {
+    if (nvv == 3) {
+      tx[d] = V[1][d] - V[0][d];
+      ty[d] = V[2][d] - V[0][d];
+      P.x[d] = V[0][d] + x * tx[d] + y * ty[d];
+    } else {
+      const double tw = V[3][d] - V[1][d] - V[2][d] + V[0][d];
+      tx[d] = V[1][d] - V[0][d] + y * tw;
+      ty[d] = V[2][d] - V[0][d] + x * tw;
+      P.x[d] = V[0][d] + x * (V[1][d] - V[0][d]) + y * (V[2][d] - V[0][d]) + x * y * tw;
+    }
+  }
+  const double cr[3] = {tx[1] * ty[2] - tx[2] * ty[1], tx[2] * ty[0] - tx[0] * ty[2], tx[0] * ty[1] - tx[1] * ty[0]};
+  P.jdet = std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+  for (int d = 0; d < 3; ++d) P.n[d] = cr[d] / P.jdet;
+  double val[4] = {1, 0, 0, 0}, dx[4] = {0, 0, 0, 0}, dy[4] = {0, 0, 0, 0};
+  if (degree == 1 && nvv == 3) {
+    val[0] = 1 - x - y; val[1] = x; val[2] = y;
+    dx[0] = -1; dx[1] = 1; dx[2] = 0;
+    dy[0] = -1; dy[1] = 0; dy[2] = 1;
+  } else if (degree == 1) {
+    val[0] = (1 - x) * (1 - y); val[1] = x * (1 - y); val[2] = (1 - x) * y; val[3] = x * y;
+    dx[0] = -(1 - y); dx[1] = 1 - y; dx[2] = -y; dx[3] = y;
+    dy[0] = -(1 - x); dy[1] = -x; dy[2] = 1 - x; dy[3] = x;
+  }
+  for (int b = 0; b < 4; ++b) {
+    P.phi[b] = val[b];
+    for (int d = 0; d < 3; ++d) P.curl[b][d] = ty[d] * dx[b] - tx[d] * dy[b];
+  }
+  return P;
+}
+
+// kernel value G and grad_y G at (x, y) (green-kernels' convention, 1/(4 pi) included)
+template <typename T>
+void kernel_at(bool helm, double k, const double* x, const double* y, T* kv) {
+  const double d[3] = {x[0] - y[0], x[1] - y[1], x[2] - y[2]};
+  const double r = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  const double g0 = 1.0 / (4.0 * kPi * r);
+  T f;
+  if (helm) {
+    const double s = std::sin(k * r), co = std::cos(k * r), kr = k * r;
+    kv[0] = Scalar<T>::make(co * g0, s * g0);
+    f = Scalar<T>::make((co + kr * s) * g0 / (r * r), (s - kr * co) * g0 / (r * r));
+  } else {
+    kv[0] = Scalar<T>::make(g0, 0.0);
+    f = Scalar<T>::make(g0 / (r * r), 0.0);
+  }
+  for (int m = 0; m < 3; ++m) kv[1 + m] = f * d[m];
+}
+
+// integrand at one point pair, without the weights and Jacobians
+// (src/boundary_assemblers/integrands/*.rs; HS: src/laplace.rs:65-78, src/helmholtz.rs:89-107)
+template <typename T>
+T cell_integrand(int op, bool helm, double k, const T* kv, const CPoint& P, int a, const CPoint& Q, int b) {
+  switch (op) {
+    case 0: return kv[0] * (P.phi[a] * Q.phi[b]);
+    case 1: return (kv[1] * Q.n[0] + kv[2] * Q.n[1] + kv[3] * Q.n[2]) * (P.phi[a] * Q.phi[b]);
+    case 2: return -(kv[1] * P.n[0] + kv[2] * P.n[1] + kv[3] * P.n[2]) * (P.phi[a] * Q.phi[b]);
+    default: {
+      const double cc = (P.curl[a][0] * Q.curl[b][0] + P.curl[a][1] * Q.curl[b][1] + P.curl[a][2] * Q.curl[b][2]) /
+                        (P.jdet * Q.jdet);
+      T out = kv[0] * cc;
+      if (helm) {
+        const double nn = P.n[0] * Q.n[0] + P.n[1] * Q.n[1] + P.n[2] * Q.n[2];
+        out = out + Scalar<T>::make(-k * k, 0.0) * (kv[0] * (nn * P.phi[a] * Q.phi[b]));
+      }
+      return out;
+    }
+  }
+}
+
+// regular rule of a cell type: triangles the symmetric rules, quadrilaterals m x m Gauss
+void cell_regular_rule(int nvv, int npts, std::vector<double>& p, std::vector<double>& w) {
+  if (nvv == 3) {
+    regular_rule(npts, p, w);
+    return;
+  }
+  int m = 1;
+  while (m * m < npts) ++m;
+  if (m * m != npts)
+    throw std::runtime_error("a quadrilateral rule needs a square number of points, got " + std::to_string(npts));
+  std::vector<double> g, gw;
+  gauss_legendre01(m, g, gw);
+  p.clear();
+  w.clear();
+  for (int j = 0; j < m; ++j)
+    for (int i = 0; i < m; ++i) {
+      p.push_back(g[i]);
+      p.push_back(g[j]);
+      w.push_back(gw[i] * gw[j]);
+    }
+}
+
+// reference corners of a cell type
+const double kCorner[4][2] = {{0, 0}, {1, 0}, {0, 1}, {1, 1}};
+
+int quad_opposite(int v) { return 3 - v; }
+
+// sub-triangles (local vertex triples) of cell c for the shared local vertices `sh`
+// (|sh| = 1 or 2; all vertices for a coincident pair)
+std::vector<std::array<int, 3>> sub_triangles(int nvv, const std::vector<int>& sh) {
+  if (nvv == 3) return {{0, 1, 2}};
+  if (sh.size() >= 3) return {{0, 1, 3}, {0, 3, 2}};
+  const int s1 = sh[0], o1 = quad_opposite(s1);
+  int others[2], n = 0;
+  for (int v = 0; v < 4; ++v)
+    if (v != s1 && v != o1) others[n++] = v;
+  if (sh.size() == 1) return {{s1, others[0], o1}, {s1, o1, others[1]}};
+  const int s2 = sh[1];
+  if (s2 == o1) throw std::runtime_error("two quadrilaterals share opposite vertices but not an edge");
+  const int t = others[0] == s2 ? others[1] : others[0];
+  return {{s1, s2, o1}, {s1, o1, t}};
+}
+
+// Duffy rule of an adjacent cell pair in the two cells' own reference coordinates
+PairRule cell_pair_rule(const CGrid& g, int64_t tc, int64_t sc, int nsing) {
+  std::vector<int> tsh, ssh;  // shared local vertices, in trial-local order of the shared vertex
+  for (int si = 0; si < g.nvc[sc]; ++si)
+    for (int ti = 0; ti < g.nvc[tc]; ++ti)
+      if (g.v(tc, ti) == g.v(sc, si)) {
+        tsh.push_back(ti);
+        ssh.push_back(si);
+      }
+  if (tc == sc) {
+    tsh.clear();
+    ssh.clear();
+    for (int m = 0; m < g.nvc[tc]; ++m) { tsh.push_back(m); ssh.push_back(m); }
+  }
+  const auto tsub = sub_triangles(g.nvc[tc], tsh), ssub = sub_triangles(g.nvc[sc], ssh);
+  PairRule out;
+  for (const auto& A : tsub)
+    for (const auto& B : ssub) {
+      std::vector<std::pair<int, int>> pairs;  // (test sub-local, trial sub-local), trial order
+      for (int bi = 0; bi < 3; ++bi)
+        for (int ai = 0; ai < 3; ++ai)
+          if (g.v(tc, A[ai]) == g.v(sc, B[bi])) pairs.push_back({ai, bi});
+      if (pairs.empty()) throw std::runtime_error("sub-triangle pair without a common vertex");
+      const PairRule R = duffy_rule(pairs, nsing);
+      // affine maps reference triangle -> sub-triangle of each cell's reference element
+      double ta[2][3], sa[2][3];  // [coordinate][origin, d0, d1]
+      for (int d = 0; d < 2; ++d) {
+        ta[d][0] = kCorner[A[0]][d];
+        ta[d][1] = kCorner[A[1]][d] - kCorner[A[0]][d];
+        ta[d][2] = kCorner[A[2]][d] - kCorner[A[0]][d];
+        sa[d][0] = kCorner[B[0]][d];
+        sa[d][1] = kCorner[B[1]][d] - kCorner[B[0]][d];
+        sa[d][2] = kCorner[B[2]][d] - kCorner[B[0]][d];
+      }
+      const double tdet = std::fabs(ta[0][1] * ta[1][2] - ta[0][2] * ta[1][1]);
+      const double sdet = std::fabs(sa[0][1] * sa[1][2] - sa[0][2] * sa[1][1]);
+      for (size_t i = 0; i < R.w.size(); ++i) {
+        for (int d = 0; d < 2; ++d) {
+          out.tp.push_back(ta[d][0] + ta[d][1] * R.tp[2 * i] + ta[d][2] * R.tp[2 * i + 1]);
+          out.sp.push_back(sa[d][0] + sa[d][1] * R.sp[2 * i] + sa[d][2] * R.sp[2 * i + 1]);
+        }
+        out.w.push_back(R.w[i] * tdet * sdet);
+      }
+    }
+  return out;
+}
+
+bool cells_touch(const CGrid& g, int64_t a, int64_t b) {
+  for (int i = 0; i < g.nvc[a]; ++i)
+    for (int j = 0; j < g.nvc[b]; ++j)
+      if (g.v(a, i) == g.v(b, j)) return true;
+  return false;
+}
+
+// dense test x trial matrix (column-major) = regular part + singular part
+template <typename T>
+void assemble_cells_t(int kind, int op, double k, const CGrid& g, const CSpace& test, const CSpace& trial, int nq_tri,
+                      int nq_quad, int nsing, int nthreads, T* out) {
+  const bool helm = kind == 1;
+  std::vector<double> rp[2], rw[2];
+  cell_regular_rule(3, nq_tri, rp[0], rw[0]);
+  cell_regular_rule(4, nq_quad, rp[1], rw[1]);
+  const int64_t ld = test.size;
+  std::vector<std::vector<CPoint>> tpts(g.nc), spts(g.nc);
+  for (int64_t c = 0; c < g.nc; ++c) {
+    const int r = g.nvc[c] == 3 ? 0 : 1;
+    for (size_t p = 0; p < rw[r].size(); ++p) {
+      tpts[c].push_back(cell_point(g, c, test.degree, rp[r][2 * p], rp[r][2 * p + 1]));
+      spts[c].push_back(cell_point(g, c, trial.degree, rp[r][2 * p], rp[r][2 * p + 1]));
+    }
+  }
+  // rows of one test cell are written by one thread: split the work by test DOF owner
+  // through a per-thread output, summed at the end
+  const int nt = std::max(1, nthreads);
+  std::vector<std::vector<T>> part(nt, std::vector<T>((size_t)ld * trial.size, T(0.0)));
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nt)
+  for (int64_t tc = 0; tc < g.nc; ++tc) {
+    T* o = part[omp_get_thread_num()].data();
+    const int rt = g.nvc[tc] == 3 ? 0 : 1;
+    for (int64_t sc = 0; sc < g.nc; ++sc) {
+      T loc[4][4];
+      for (auto& row : loc)
+        for (auto& e : row) e = T(0.0);
+      if (!cells_touch(g, tc, sc)) {
+        const int rs = g.nvc[sc] == 3 ? 0 : 1;
+        for (size_t p = 0; p < rw[rt].size(); ++p) {
+          const CPoint& P = tpts[tc][p];
+          for (size_t q = 0; q < rw[rs].size(); ++q) {
+            const CPoint& Q = spts[sc][q];
+            T kv[4];
+            kernel_at<T>(helm, k, P.x, Q.x, kv);
+            const double wt = rw[rt][p] * P.jdet * rw[rs][q] * Q.jdet;
+            for (int a = 0; a < test.nb(tc); ++a)
+              for (int b = 0; b < trial.nb(sc); ++b) loc[a][b] += cell_integrand<T>(op, helm, k, kv, P, a, Q, b) * wt;
+          }
+        }
+      } else {
+        const PairRule R = cell_pair_rule(g, tc, sc, nsing);
+        for (size_t i = 0; i < R.w.size(); ++i) {
+          const CPoint P = cell_point(g, tc, test.degree, R.tp[2 * i], R.tp[2 * i + 1]);
+          const CPoint Q = cell_point(g, sc, trial.degree, R.sp[2 * i], R.sp[2 * i + 1]);
+          T kv[4];
+          kernel_at<T>(helm, k, P.x, Q.x, kv);
+          const double wt = R.w[i] * P.jdet * Q.jdet;
+          for (int a = 0; a < test.nb(tc); ++a)
+            for (int b = 0; b < trial.nb(sc); ++b) loc[a][b] += cell_integrand<T>(op, helm, k, kv, P, a, Q, b) * wt;
+        }
+      }
+      for (int a = 0; a < test.nb(tc); ++a)
+        for (int b = 0; b < trial.nb(sc); ++b)
+          o[test.dofs[test.off[tc] + a] + ld * trial.dofs[trial.off[sc] + b]] += loc[a][b];
+    }
+  }
+  for (int t = 0; t < nt; ++t)
+    for (size_t i = 0; i < part[t].size(); ++i) out[i] += part[t][i];
+}
+
+}  // namespace
+
+extern "C" {
+
+// Dense assembly on a grid of triangle and quadrilateral cells, accumulated into `out`
+// (column-major test x trial; complex interleaved for Helmholtz). cell_vertices: the
+// cells' vertex lists back to back (3 per triangle, 4 per quadrilateral in tensor order);
+// cell_types: 2 triangle, 3 quadrilateral. nq_tri / nq_quad: regular points per cell type.
+int oracle_assemble_cells(int kind, int op, double k, const double* pts, int64_t nv, const int64_t* cell_vertices,
+                          const uint8_t* cell_types, int64_t nc, int test_degree, int test_cont, int trial_degree,
+                          int trial_cont, int nq_tri, int nq_quad, int nsing, int nthreads, void* out) {
+  try {
+    CGrid g(pts, nv, cell_vertices, cell_types, nc);
+    CSpace ts(&g, test_degree, test_cont != 0), ss(&g, trial_degree, trial_cont != 0);
+    if (kind == 0) assemble_cells_t<double>(kind, op, k, g, ts, ss, nq_tri, nq_quad, nsing, nthreads, (double*)out);
+    else assemble_cells_t<cplx>(kind, op, k, g, ts, ss, nq_tri, nq_quad, nsing, nthreads, (cplx*)out);
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+// Space on such a grid: *size and the cells' DOFs back to back (1 per cell for degree 0,
+// one per vertex for degree 1).
+int oracle_cells_space(const double* pts, int64_t nv, const int64_t* cell_vertices, const uint8_t* cell_types,
+                       int64_t nc, int degree, int cont, int64_t* size, int64_t* dofs) {
+  try {
+    CGrid g(pts, nv, cell_vertices, cell_types, nc);
+    CSpace s(&g, degree, cont != 0);
+    *size = s.size;
+    if (dofs) std::memcpy(dofs, s.dofs.data(), s.dofs.size() * sizeof(int64_t));
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+// The Duffy rule of one adjacent cell pair (tests): count in *n, then the points.
+int oracle_cell_pair_rule(const double* pts, int64_t nv, const int64_t* cell_vertices, const uint8_t* cell_types,
+                          int64_t nc, int64_t test_cell, int64_t trial_cell, int nsing, int64_t* n, double* test_pts,
+                          double* trial_pts, double* weights) {
+  try {
+    CGrid g(pts, nv, cell_vertices, cell_types, nc);
+    const PairRule R = cell_pair_rule(g, test_cell, trial_cell, nsing);
+    *n = (int64_t)R.w.size();
+    if (test_pts) {
+      std::memcpy(test_pts, R.tp.data(), R.tp.size() * sizeof(double));
+      std::memcpy(trial_pts, R.sp.data(), R.sp.size() * sizeof(double));
+      std::memcpy(weights, R.w.data(), R.w.size() * sizeof(double));
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
   }
 }
 

@@ -58,6 +58,10 @@ def lib():
         L.oracle_gauss_legendre01.argtypes = [i32, vp, vp]
         L.oracle_duffy_rule.argtypes = [i32, vp, vp, i32, vp, vp, vp]
         L.oracle_potential.argtypes = [i32, i32, dbl, vp, i64, vp, i64, i32, i32, i32, vp, i64, vp]
+        L.oracle_assemble_cells.argtypes = [i32, i32, dbl, vp, i64, vp, vp, i64, i32, i32, i32, i32, i32, i32, i32,
+                                            i32, vp]
+        L.oracle_cells_space.argtypes = [vp, i64, vp, vp, i64, i32, i32, vp, vp]
+        L.oracle_cell_pair_rule.argtypes = [vp, i64, vp, vp, i64, i64, i64, i32, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -173,4 +177,60 @@ def duffy_rule(pairs, n):
     sp = np.zeros((npts, 2))
     w = np.zeros(npts)
     lib().oracle_duffy_rule(len(pairs), _p(tl), _p(sl), n, _p(tp), _p(sp), _p(w))
+    return tp, sp, w
+
+
+def _cells_arrays(cells):
+    """cells: a list of vertex lists (3: triangle, 4: quadrilateral in tensor order) or an
+    [n, 3] / [n, 4] array -> (flat vertices, types)."""
+    cl = [list(map(int, c)) for c in cells]
+    flat = np.array([v for c in cl for v in c], dtype=np.int64)
+    types = np.array([2 if len(c) == 3 else 3 for c in cl], dtype=np.uint8)
+    if any(len(c) not in (3, 4) for c in cl):
+        raise ValueError("cells must have 3 or 4 vertices")
+    return flat, types
+
+
+def cells_space(pts, cells, degree, continuous):
+    """(size, per-cell DOF lists) of a space on a grid of triangles and quadrilaterals."""
+    pts = np.ascontiguousarray(pts, dtype=np.float64)
+    flat, types = _cells_arrays(cells)
+    size = ctypes.c_int64()
+    n = len(types) if degree == 0 else len(flat)
+    dofs = np.zeros(n, dtype=np.int64)
+    _check(lib().oracle_cells_space(_p(pts), len(pts), _p(flat), _p(types), len(types), degree, int(continuous),
+                                    ctypes.byref(size), _p(dofs)))
+    out, i = [], 0
+    for t in types:
+        nb = 1 if degree == 0 else (3 if t == 2 else 4)
+        out.append([int(x) for x in dofs[i:i + nb]])
+        i += nb
+    return size.value, out
+
+
+def assemble_cells(kind, op, pts, cells, test=(0, False), trial=(0, False), k=0.0, nq_tri=6, nq_quad=16, nsing=4,
+                   nthreads=0):
+    """Dense assembly on a grid of triangles and quadrilaterals (the general-cell oracle)."""
+    pts = np.ascontiguousarray(pts, dtype=np.float64)
+    flat, types = _cells_arrays(cells)
+    nt = cells_space(pts, cells, *test)[0]
+    ns = cells_space(pts, cells, *trial)[0]
+    dtype = np.float64 if kind == "laplace" else np.complex128
+    out = np.zeros((nt, ns), dtype=dtype, order="F")
+    _check(lib().oracle_assemble_cells(KINDS[kind], OPS[op], float(k), _p(pts), len(pts), _p(flat), _p(types),
+                                       len(types), test[0], int(test[1]), trial[0], int(trial[1]), nq_tri, nq_quad,
+                                       nsing, nthreads or os.cpu_count() or 1, _p(out)))
+    return out
+
+
+def cell_pair_rule(pts, cells, test_cell, trial_cell, nsing):
+    """Duffy rule of an adjacent cell pair in the cells' reference coordinates: (test points,
+    trial points, weights)."""
+    pts = np.ascontiguousarray(pts, dtype=np.float64)
+    flat, types = _cells_arrays(cells)
+    n = ctypes.c_int64()
+    args = (_p(pts), len(pts), _p(flat), _p(types), len(types), int(test_cell), int(trial_cell), nsing)
+    _check(lib().oracle_cell_pair_rule(*args, ctypes.byref(n), None, None, None))
+    tp, sp, w = np.zeros((n.value, 2)), np.zeros((n.value, 2)), np.zeros(n.value)
+    _check(lib().oracle_cell_pair_rule(*args, ctypes.byref(n), _p(tp), _p(sp), _p(w)))
     return tp, sp, w
