@@ -40,6 +40,10 @@ _SIGNATURES = {
     "grid_ncells": (c_size, [c_vp]),
     "grid_points_copy": (None, [c_vp, c_vp]),
     "grid_cells_copy": (None, [c_vp, c_vp]),
+    "grid_new_cells": (c_vp, [c_vp, c_size, c_vp, c_vp, c_size]),
+    "grid_cell_vertices_size": (c_size, [c_vp]),
+    "grid_cell_vertices_copy": (None, [c_vp, c_vp]),
+    "grid_cell_types_copy": (None, [c_vp, c_vp]),
     "grid_set_points": (c_int, [c_vp, c_vp]),
     "function_space_new": (c_vp, [c_vp, c_size, c_u8]),
     "space_t_free": (None, [c_vp]),

@@ -152,6 +152,35 @@ void grid_points_copy(const GridT* g, double* p) {
 void grid_cells_copy(const GridT* g, int64_t* c) {
   if (g && c) std::copy(g->g->cells.begin(), g->g->cells.end(), c);
 }
+GridT* grid_new_cells(const double* points, size_t npoints, const int64_t* cell_vertices, const uint8_t* cell_types,
+                      size_t ncells) {
+  return guarded(
+      [&]() -> GridT* {
+        need(points, "points");
+        need(cell_types, "cell types");
+        size_t nverts = 0;
+        for (size_t c = 0; c < ncells; ++c) nverts += cell_types[c] == BEMPP_CELL_QUADRILATERAL ? 4 : 3;
+        if (nverts) need(cell_vertices, "cell vertices");
+        std::vector<double> p(points, points + 3 * npoints);
+        std::vector<int64_t> v(cell_vertices, cell_vertices + nverts);
+        std::vector<uint8_t> t(cell_types, cell_types + ncells);
+        return new GridT{new Grid(std::move(p), std::move(v), std::move(t))};
+      },
+      (GridT*)nullptr);
+}
+size_t grid_cell_vertices_size(const GridT* g) {
+  if (!g) return 0;
+  return g->g->general() ? g->g->cell_verts.size() : g->g->cells.size();
+}
+void grid_cell_vertices_copy(const GridT* g, int64_t* v) {
+  if (!g || !v) return;
+  const auto& src = g->g->general() ? g->g->cell_verts : g->g->cells;
+  std::copy(src.begin(), src.end(), v);
+}
+void grid_cell_types_copy(const GridT* g, uint8_t* t) {
+  if (!g || !t) return;
+  for (int64_t c = 0; c < g->g->nc(); ++c) t[c] = g->g->cell_nv(c) == 4 ? BEMPP_CELL_QUADRILATERAL : BEMPP_CELL_TRIANGLE;
+}
 int grid_set_points(GridT* g, const double* p) {
   return guarded(
       [&]() {
@@ -182,10 +211,12 @@ GridT* single_element_grid_borrowed_create_bempp(size_t tdim, const size_t*, con
         need(cells, "cells");
         if (tdim != 2) throw Error("only grids of topological dimension 2 are supported, got " + std::to_string(tdim));
         if (gdim != 3) throw Error("only grids in 3 dimensions are supported, got " + std::to_string(gdim));
-        if (geometry_degree != 1 || points_per_cell != 3)
-          throw Error("only flat triangles (geometry degree 1, 3 points per cell) are supported");
-        if (entity_types && entity_types[2] != BEMPP_CELL_TRIANGLE)
-          throw Error("only triangle cells are supported, got cell type " + std::to_string(entity_types[2]));
+        const uint8_t ctype = points_per_cell == 4 ? BEMPP_CELL_QUADRILATERAL : BEMPP_CELL_TRIANGLE;
+        if (geometry_degree != 1 || (points_per_cell != 3 && points_per_cell != 4))
+          throw Error("only degree-1 geometry is supported (3 points per triangle, 4 per quadrilateral)");
+        if (entity_types && entity_types[2] != ctype)
+          throw Error("cell type " + std::to_string(entity_types[2]) + " does not match " +
+                      std::to_string(points_per_cell) + " points per cell (triangle or quadrilateral cells only)");
         if (dtype != BEMPP_DTYPE_F32 && dtype != BEMPP_DTYPE_F64)
           throw Error("grid points must be f32 or f64, got dtype " + std::to_string(dtype));
         if (entity_counts && entity_counts[2] != ncells)
@@ -193,29 +224,33 @@ GridT* single_element_grid_borrowed_create_bempp(size_t tdim, const size_t*, con
         std::vector<double> p(3 * npoints);
         for (size_t i = 0; i < 3 * npoints; ++i)
           p[i] = dtype == BEMPP_DTYPE_F64 ? ((const double*)points)[i] : (double)((const float*)points)[i];
-        std::vector<int64_t> c(3 * ncells);
-        for (size_t i = 0; i < 3 * ncells; ++i) {
-          if (cells[i] >= npoints) throw Error("cell " + std::to_string(i / 3) + " refers to a missing point");
+        const size_t ppc = points_per_cell;
+        std::vector<int64_t> c(ppc * ncells);
+        for (size_t i = 0; i < ppc * ncells; ++i) {
+          if (cells[i] >= npoints) throw Error("cell " + std::to_string(i / ppc) + " refers to a missing point");
           c[i] = (int64_t)cells[i];
         }
         std::vector<int64_t> topo;
         const size_t* tc = downward_connectivity && downward_connectivity[2] ? downward_connectivity[2][0] : nullptr;
         if (tc) {
-          if (downward_connectivity_shape0 && downward_connectivity_shape0[2] && downward_connectivity_shape0[2][0] != 3)
-            throw Error("cell-to-vertex connectivity must have 3 vertices per cell");
+          if (downward_connectivity_shape0 && downward_connectivity_shape0[2] &&
+              downward_connectivity_shape0[2][0] != ppc)
+            throw Error("cell-to-vertex connectivity must have " + std::to_string(ppc) + " vertices per cell");
           const size_t nvert = entity_counts ? entity_counts[0] : npoints;
           topo.assign(nvert, -1);
           std::vector<int64_t> point_topo(npoints, -1);
-          for (size_t i = 0; i < 3 * ncells; ++i) {
+          for (size_t i = 0; i < ppc * ncells; ++i) {
             const size_t v = tc[i];
-            if (v >= nvert) throw Error("cell " + std::to_string(i / 3) + " refers to a missing vertex");
+            if (v >= nvert) throw Error("cell " + std::to_string(i / ppc) + " refers to a missing vertex");
             if ((topo[v] >= 0 && topo[v] != c[i]) || (point_topo[c[i]] >= 0 && point_topo[c[i]] != (int64_t)v))
               throw Error("topology and geometry cells differ by more than a relabelling of the vertices");
             topo[v] = c[i];
             point_topo[c[i]] = (int64_t)v;
           }
         }
-        auto* g = new GridT{new Grid(std::move(p), std::move(c)), std::move(topo)};
+        Grid* grid = ppc == 3 ? new Grid(std::move(p), std::move(c))
+                              : new Grid(std::move(p), std::move(c), std::vector<uint8_t>(ncells, ctype));
+        auto* g = new GridT{grid, std::move(topo)};
         return g;
       },
       (GridT*)nullptr);
@@ -253,7 +288,8 @@ SpaceT* function_space_new(const GridT* g, size_t degree, uint8_t continuity) {
         const FunctionSpace& fs = *sp->s;
         if (fs.continuous && fs.degree == 1) {
           sp->point_dof.assign(fs.grid->nv(), -1);
-          for (size_t i = 0; i < fs.dofs.size(); ++i) sp->point_dof[fs.grid->cells[i]] = fs.dofs[i];
+          for (int64_t c = 0; c < fs.grid->nc(); ++c)
+            for (int m = 0; m < fs.cell_nb(c); ++m) sp->point_dof[fs.grid->vertex(c, m)] = fs.dofs[(size_t)fs.nb * c + m];
         }
         return sp;
       },
@@ -300,7 +336,7 @@ const int64_t* entity_dofs(const SpaceT* s, size_t dim, size_t index, size_t* n)
     return &s->point_dof[point];
   }
   if (dim != 2 || index >= (size_t)fs.grid->nc()) return nullptr;
-  *n = (size_t)fs.nb;
+  *n = (size_t)fs.cell_nb((int64_t)index);
   return &fs.dofs[index * fs.nb];
 }
 }  // namespace
@@ -325,11 +361,11 @@ size_t space_local_size(const SpaceT* s) { return s ? (size_t)s->s->size : 0; }
 int space_is_serial(const SpaceT* s) { return s ? 1 : 0; }
 int space_has_cell_dofs(const SpaceT* s, size_t cell) { return s && cell < (size_t)s->s->grid->nc() ? 1 : 0; }
 size_t space_cell_dofs_size(const SpaceT* s, size_t cell) {
-  return space_has_cell_dofs(s, cell) ? (size_t)s->s->nb : 0;
+  return space_has_cell_dofs(s, cell) ? (size_t)s->s->cell_nb((int64_t)cell) : 0;
 }
 void space_cell_dofs(const SpaceT* s, size_t cell, uintptr_t* dofs) {
   if (!space_has_cell_dofs(s, cell) || !dofs) return;
-  for (int b = 0; b < s->s->nb; ++b) dofs[b] = (uintptr_t)s->s->dofs[cell * s->s->nb + b];
+  for (int b = 0; b < s->s->cell_nb((int64_t)cell); ++b) dofs[b] = (uintptr_t)s->s->dofs[cell * s->s->nb + b];
 }
 size_t space_global_dof_index(const SpaceT*, size_t local) { return local; }
 size_t space_cell_colouring(const SpaceT* s, int64_t* colour) {

@@ -59,14 +59,24 @@ void free_device_space_cache(DeviceSpaceCache*);
 // ---------------------------------------------------------------------------
 struct Grid {
   std::vector<double> points;  // [nv][3]
-  std::vector<int64_t> cells;  // [nc][3]
+  std::vector<int64_t> cells;  // [nc][3] (triangle grids; empty for grids with quadrilaterals)
   std::vector<int64_t> vc_offsets, vc_cells;  // vertex -> cells, ascending
   uint64_t version = 0;  // bumped when points change
+  // Grids with quadrilateral cells (single-element or mixed, degree-1 geometry): per cell its
+  // type (kTriangle or kQuadrilateral) and its vertices back to back from cell_off[c]
+  // (quadrilaterals in tensor order: reference (0,0), (1,0), (0,1), (1,1)). Empty for
+  // triangle grids; a grid of triangles only is stored as a triangle grid.
+  std::vector<uint8_t> types;
+  std::vector<int64_t> cell_verts, cell_off;
 
   Grid(std::vector<double> pts, std::vector<int64_t> cls);
+  Grid(std::vector<double> pts, std::vector<int64_t> cell_vertices, std::vector<uint8_t> cell_types);
   ~Grid();
   int64_t nv() const { return (int64_t)points.size() / 3; }
-  int64_t nc() const { return (int64_t)cells.size() / 3; }
+  int64_t nc() const { return general() ? (int64_t)types.size() : (int64_t)cells.size() / 3; }
+  bool general() const { return !types.empty(); }
+  int cell_nv(int64_t c) const { return general() && types[c] == kQuadrilateral ? 4 : 3; }
+  int64_t vertex(int64_t c, int m) const { return general() ? cell_verts[cell_off[c] + m] : cells[3 * c + m]; }
   void set_points(const double* pts);
 
   std::mutex cache_mutex;
@@ -84,11 +94,12 @@ struct FunctionSpace {
   const Grid* grid;
   int degree;
   bool continuous;
-  int nb;                        // basis functions per cell
-  std::vector<int64_t> dofs;     // [nc][nb]
+  int nb;                        // basis functions per cell (the largest: 4 on quadrilaterals)
+  std::vector<int64_t> dofs;     // [nc][nb]; a triangle's fourth entry is -1
   int64_t size = 0;
 
   FunctionSpace(const Grid* g, int degree, bool continuous);
+  int cell_nb(int64_t c) const { return degree == 0 ? 1 : grid->cell_nv(c); }
   ~FunctionSpace();
   std::vector<std::vector<int64_t>> colouring() const;
 
@@ -103,6 +114,11 @@ struct FunctionSpace {
 void triangle_rule(int npoints, std::vector<double>& xi, std::vector<double>& w);
 // Gauss-Legendre on [0,1].
 void gauss01(int n, std::vector<double>& x, std::vector<double>& w);
+// m x m Gauss-Legendre rule on the reference square (npoints = m^2), weights sum 1.
+void quadrilateral_rule(int npoints, std::vector<double>& xi, std::vector<double>& w);
+// Duffy rule of an adjacent cell pair of a grid with quadrilaterals, in the two cells' own
+// reference coordinates: records (test xi0, test xi1, trial xi0, trial xi1, weight).
+void cell_pair_rule(const Grid& g, int64_t test_cell, int64_t trial_cell, int n, std::vector<double>& pts5);
 // Canonical Sauter-Schwab rule with `nshared` (1,2,3) shared vertices at the
 // canonical positions V0 (vertex), V0,V1 (edge), all (coincident); each point
 // is (test xi0, test xi1, trial xi0, trial xi1, weight); weights sum to 1/4.
@@ -141,6 +157,11 @@ Assembler* make_assembler(Kind kind, Op op, DType dtype, double k, const Options
 // Dense assembly into device memory on the current device (accumulate or overwrite).
 void assemble_device(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test, void* dev_out,
                      bool accumulate, void* stream);
+// Dense assembly on a grid with quadrilateral cells (FP64, any output dtype).
+void assemble_cells_device(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test, void* dev_out,
+                           bool accumulate, void* stream);
+// Refuses grids with quadrilateral cells in the paths specialised to triangles.
+void require_triangles(const FunctionSpace& s, const char* what);
 // Dense assembly accumulated into a host buffer.
 void assemble_host(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test, void* host_out);
 // Singular part as COO triplets (rows = test DOFs, cols = trial DOFs); values

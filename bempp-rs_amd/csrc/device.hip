@@ -755,6 +755,8 @@ void check_assembler(const Assembler& a, const FunctionSpace& trial, const Funct
 Prepared prepare(const Assembler& a, const FunctionSpace& trial_c, const FunctionSpace& test_c, void* out,
                  void* stream, bool caller_stream) {
   check_assembler(a, trial_c, test_c);
+  require_triangles(test_c, "this entry point");
+  require_triangles(trial_c, "this entry point");
   auto& trial = const_cast<FunctionSpace&>(trial_c);
   auto& test = const_cast<FunctionSpace&>(test_c);
   auto& tgrid = const_cast<Grid&>(*test.grid);
@@ -911,6 +913,13 @@ GradientInputs gradient_inputs(const Assembler& a, const FunctionSpace& trial, c
 void assemble_device(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test, void* dev_out,
                      bool accumulate, void* stream) {
   if (!dev_out) throw Error("output pointer is null");
+  if (test.grid->general() || trial.grid->general()) {
+    // grids with quadrilateral cells: the general-cell kernels (FP64 for every precision mode)
+    check_assembler(a, trial, test);
+    (void)current_device();
+    assemble_cells_device(a, trial, test, dev_out, accumulate, stream);
+    return;
+  }
   Prepared P = prepare(a, trial, test, dev_out, stream);
   P.stream = (hipStream_t)stream;  // device entry point: the caller's stream, NULL = the null stream
   const bool p0p0 = test.nb == 1 && trial.nb == 1;
@@ -1186,6 +1195,7 @@ void launch_potential_q(int Q, dim3 grid, hipStream_t s, const dev::PotentialArg
 
 void assemble_potential_device(const Assembler& a, const FunctionSpace& space_c, const double* dev_points,
                                int64_t npoints, void* dev_out, bool accumulate, void* stream) {
+  require_triangles(space_c, "potential assembly");
   if (a.op != kSL && a.op != kDL) throw Error("potential operators exist for the single and double layer only");
   if ((a.kind == kLaplace) == is_complex(a.dtype)) throw Error("dtype does not match the kernel");
   if (npoints < 0) throw Error("negative point count");

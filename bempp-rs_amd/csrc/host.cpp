@@ -40,6 +40,49 @@ Grid::Grid(std::vector<double> pts, std::vector<int64_t> cls) : points(std::move
     for (int m = 0; m < 3; ++m) vc_cells[fill[cells[3 * c + m]]++] = c;  // ascending c
 }
 
+Grid::Grid(std::vector<double> pts, std::vector<int64_t> cell_vertices, std::vector<uint8_t> cell_types)
+    : points(std::move(pts)) {
+  if (points.size() % 3) throw Error("grid points must have 3 coordinates each");
+  const int64_t n_v = nv(), n_c = (int64_t)cell_types.size();
+  if (n_v > INT32_MAX || n_c > INT32_MAX) throw Error("grid too large for 32-bit cell/vertex indices");
+  std::vector<int64_t> off(n_c + 1, 0);
+  bool any_quad = false;
+  for (int64_t c = 0; c < n_c; ++c) {
+    if (cell_types[c] != kTriangle && cell_types[c] != kQuadrilateral)
+      throw Error("cell " + std::to_string(c) + ": only triangle (2) and quadrilateral (3) cells are supported");
+    any_quad = any_quad || cell_types[c] == kQuadrilateral;
+    off[c + 1] = off[c] + (cell_types[c] == kTriangle ? 3 : 4);
+  }
+  if ((int64_t)cell_vertices.size() != off[n_c])
+    throw Error("cell vertex list has " + std::to_string(cell_vertices.size()) + " entries, the cell types need " +
+                std::to_string(off[n_c]));
+  for (int64_t c = 0; c < n_c; ++c) {
+    const int nvv = (int)(off[c + 1] - off[c]);
+    for (int m = 0; m < nvv; ++m) {
+      const int64_t v = cell_vertices[off[c] + m];
+      if (v < 0 || v >= n_v) throw Error("cell " + std::to_string(c) + " has a vertex index out of range");
+      for (int q = 0; q < m; ++q)
+        if (cell_vertices[off[c] + q] == v) throw Error("cell " + std::to_string(c) + " repeats a vertex");
+    }
+  }
+  if (!any_quad) {
+    cells = std::move(cell_vertices);
+  } else {
+    types = std::move(cell_types);
+    cell_verts = std::move(cell_vertices);
+    cell_off = std::move(off);
+  }
+  std::vector<int64_t> count(n_v + 1, 0);
+  for (int64_t c = 0; c < n_c; ++c)
+    for (int m = 0; m < cell_nv(c); ++m) count[vertex(c, m) + 1]++;
+  for (int64_t v = 0; v < n_v; ++v) count[v + 1] += count[v];
+  vc_offsets = count;
+  vc_cells.assign(count[n_v], 0);
+  std::vector<int64_t> fill(count.begin(), count.end() - 1);
+  for (int64_t c = 0; c < n_c; ++c)
+    for (int m = 0; m < cell_nv(c); ++m) vc_cells[fill[vertex(c, m)]++] = c;  // ascending c
+}
+
 Grid::~Grid() {
   for (auto& kv : device_cache) free_device_grid_cache(kv.second);
 }
@@ -112,21 +155,26 @@ Grid* make_sphere(int seed, uint32_t level) {
 FunctionSpace::FunctionSpace(const Grid* g, int deg, bool cont) : grid(g), degree(deg) {
   if (deg != 0 && deg != 1) throw Error("only Lagrange degree 0 and 1 are supported");
   continuous = cont && deg == 1;  // degree 0 DOFs always sit on the cell interior
-  nb = deg == 0 ? 1 : 3;
+  nb = deg == 0 ? 1 : (g->general() ? 4 : 3);
   const int64_t n_c = g->nc();
   dofs.assign(n_c * nb, -1);
   if (continuous) {
     std::vector<int64_t> vdof(g->nv(), -1);
     for (int64_t c = 0; c < n_c; ++c)
-      for (int m = 0; m < 3; ++m) {
-        int64_t& d = vdof[g->cells[3 * c + m]];
+      for (int m = 0; m < g->cell_nv(c); ++m) {
+        int64_t& d = vdof[g->vertex(c, m)];
         if (d < 0) d = size++;
-        dofs[3 * c + m] = d;
+        dofs[nb * c + m] = d;
       }
   } else {
-    for (int64_t i = 0; i < n_c * nb; ++i) dofs[i] = i;
-    size = n_c * nb;
+    for (int64_t c = 0; c < n_c; ++c)
+      for (int b = 0; b < cell_nb(c); ++b) dofs[nb * c + b] = size++;
   }
+}
+
+void require_triangles(const FunctionSpace& s, const char* what) {
+  if (s.grid->general())
+    throw Error(std::string(what) + " is not supported on grids with quadrilateral cells (dense assembly is)");
 }
 
 FunctionSpace::~FunctionSpace() {
@@ -141,10 +189,10 @@ std::vector<std::vector<int64_t>> FunctionSpace::colouring() const {
   const int64_t n_c = grid->nc();
   std::vector<std::vector<int>> held(continuous ? grid->nv() : n_c);
   for (int64_t c = 0; c < n_c; ++c) {
-    int64_t ents[3];
+    int64_t ents[4];
     int ne = 0;
     if (continuous)
-      for (int m = 0; m < 3; ++m) ents[ne++] = grid->cells[3 * c + m];
+      for (int m = 0; m < grid->cell_nv(c); ++m) ents[ne++] = grid->vertex(c, m);
     else
       ents[ne++] = c;
     size_t col = 0;
@@ -391,6 +439,118 @@ void adjacent_pairs(const Grid& g, std::vector<AdjacentPair>& out) {
       }
     }
   }
+}
+
+// m x m Gauss-Legendre on the reference square (ndelement's quadrilateral rule)
+void quadrilateral_rule(int npoints, std::vector<double>& xi, std::vector<double>& w) {
+  int m = 1;
+  while (m * m < npoints) ++m;
+  if (m * m != npoints)
+    throw Error("a quadrilateral regular rule needs a square number of points (m x m Gauss), got " +
+                std::to_string(npoints));
+  std::vector<double> g, gw;
+  gauss01(m, g, gw);
+  xi.clear();
+  w.clear();
+  for (int j = 0; j < m; ++j)
+    for (int i = 0; i < m; ++i) {
+      xi.push_back(g[i]);
+      xi.push_back(g[j]);
+      w.push_back(gw[i] * gw[j]);
+    }
+}
+
+namespace {
+// Corners of the reference cells: the triangle's are the square's first three.
+const double kRefCorner[4][2] = {{0.0, 0.0}, {1.0, 0.0}, {0.0, 1.0}, {1.0, 1.0}};
+
+// The triangles a cell is integrated over for a singular pair, as local vertex triples: a
+// triangle is itself; a quadrilateral is split along a diagonal so that both halves hold the
+// pair's first shared vertex s (and, for a shared edge (s, s2), one half holds the edge):
+// shared vertex: the diagonal through s; shared edge: the diagonal through s, halves
+// (s, s2, opposite(s)) and (s, opposite(s), the fourth vertex); coincident: the diagonal 0-3.
+// Every (test half, trial half) pair then shares s, so the triangle Sauter-Schwab rules
+// apply to it through the composite map.
+int split_cell(int nvv, const int* shared, int nshared, int (*tri)[3]) {
+  if (nvv == 3) {
+    tri[0][0] = 0, tri[0][1] = 1, tri[0][2] = 2;
+    return 1;
+  }
+  if (nshared >= 3) {
+    const int t[2][3] = {{0, 1, 3}, {0, 3, 2}};
+    std::copy(&t[0][0], &t[0][0] + 6, &tri[0][0]);
+    return 2;
+  }
+  const int s = shared[0], o = 3 - s;  // tensor order: 0-3 and 1-2 are the diagonals
+  int rest[2], nr = 0;
+  for (int v = 0; v < 4; ++v)
+    if (v != s && v != o) rest[nr++] = v;
+  if (nshared == 1) {
+    const int t[2][3] = {{s, rest[0], o}, {s, o, rest[1]}};
+    std::copy(&t[0][0], &t[0][0] + 6, &tri[0][0]);
+    return 2;
+  }
+  const int s2 = shared[1];
+  if (s2 == o) throw Error("two quadrilaterals share opposite vertices without sharing an edge");
+  const int t4 = rest[0] == s2 ? rest[1] : rest[0];
+  const int t[2][3] = {{s, s2, o}, {s, o, t4}};
+  std::copy(&t[0][0], &t[0][0] + 6, &tri[0][0]);
+  return 2;
+}
+}  // namespace
+
+void cell_pair_rule(const Grid& g, int64_t tc, int64_t sc, int n, std::vector<double>& pts5) {
+  pts5.clear();
+  const int tn = g.cell_nv(tc), sn = g.cell_nv(sc);
+  int tsh[4], ssh[4], m = 0;
+  if (tc == sc) {
+    for (int v = 0; v < tn; ++v) tsh[m] = ssh[m] = v, ++m;
+  } else {
+    for (int si = 0; si < sn; ++si)  // shared vertices in trial-local order
+      for (int ti = 0; ti < tn; ++ti)
+        if (g.vertex(tc, ti) == g.vertex(sc, si)) tsh[m] = ti, ssh[m] = si, ++m;
+  }
+  if (m == 0) throw Error("cell pair without a common vertex");
+  int ttri[2][3], stri[2][3];
+  const int nt = split_cell(tn, tsh, m, ttri), ns = split_cell(sn, ssh, m, stri);
+  std::vector<double> canon[4];
+  for (int a = 0; a < nt; ++a)
+    for (int b = 0; b < ns; ++b) {
+      // shared vertices of the two halves, in the trial half's local order
+      int tl[3], sl[3], k = 0;
+      for (int bi = 0; bi < 3; ++bi)
+        for (int ai = 0; ai < 3; ++ai)
+          if (g.vertex(tc, ttri[a][ai]) == g.vertex(sc, stri[b][bi])) tl[k] = ai, sl[k] = bi, ++k;
+      if (k == 0) throw Error("sub-triangle pair without a common vertex");
+      if (canon[k].empty()) canonical_pair_rule(k, n, canon[k]);
+      // canonical vertex j -> half-local vertex perm[j]: shared ones first, then the others
+      int tp[3], sp[3];
+      bool tu[3] = {false, false, false}, su[3] = {false, false, false};
+      for (int j = 0; j < k; ++j) tp[j] = tl[j], sp[j] = sl[j], tu[tl[j]] = su[sl[j]] = true;
+      for (int x = 0, jt = k, js = k; x < 3; ++x) {
+        if (!tu[x]) tp[jt++] = x;
+        if (!su[x]) sp[js++] = x;
+      }
+      // reference coordinates of the half's vertices in the cell's reference element
+      double tv[3][2], sv[3][2];
+      for (int j = 0; j < 3; ++j)
+        for (int d = 0; d < 2; ++d) tv[j][d] = kRefCorner[ttri[a][j]][d], sv[j][d] = kRefCorner[stri[b][j]][d];
+      const double tdet = std::fabs((tv[1][0] - tv[0][0]) * (tv[2][1] - tv[0][1]) -
+                                    (tv[2][0] - tv[0][0]) * (tv[1][1] - tv[0][1]));
+      const double sdet = std::fabs((sv[1][0] - sv[0][0]) * (sv[2][1] - sv[0][1]) -
+                                    (sv[2][0] - sv[0][0]) * (sv[1][1] - sv[0][1]));
+      const std::vector<double>& C = canon[k];
+      for (size_t i = 0; i < C.size() / 5; ++i) {
+        const double* r = &C[5 * i];
+        // barycentric coordinates of the canonical point, moved to the half's local vertices
+        const double tl3[3] = {1.0 - r[0] - r[1], r[0], r[1]}, sl3[3] = {1.0 - r[2] - r[3], r[2], r[3]};
+        double tb[3], sb[3];
+        for (int j = 0; j < 3; ++j) tb[tp[j]] = tl3[j], sb[sp[j]] = sl3[j];
+        for (int d = 0; d < 2; ++d) pts5.push_back(tb[0] * tv[0][d] + tb[1] * tv[1][d] + tb[2] * tv[2][d]);
+        for (int d = 0; d < 2; ++d) pts5.push_back(sb[0] * sv[0][d] + sb[1] * sv[1][d] + sb[2] * sv[2][d]);
+        pts5.push_back(r[4] * tdet * sdet);
+      }
+    }
 }
 
 }  // namespace bempp_amd

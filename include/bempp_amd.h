@@ -124,16 +124,33 @@ void grid_t_free(GridT* grid);
 size_t grid_npoints(const GridT* grid);
 size_t grid_ncells(const GridT* grid);
 void grid_points_copy(const GridT* grid, double* points);
+/* Triangle grids: cells [ncells][3]. (A grid with quadrilateral cells copies nothing here;
+ * use grid_cell_vertices_copy.) */
 void grid_cells_copy(const GridT* grid, int64_t* cells);
+/* A grid of triangle and quadrilateral cells (ndgrid's single-element quadrilateral grids and
+ * mixed grids, degree-1 geometry; no counterpart binding in the reference, which builds them
+ * through ndgrid's MixedGridBuilder, tests/mixed_assembly.rs.bak:15-31): points [npoints][3]
+ * f64; cell_types[c] BEMPP_CELL_TRIANGLE (3 vertices) or BEMPP_CELL_QUADRILATERAL (4
+ * vertices, tensor order: reference (0,0), (1,0), (0,1), (1,1)); cell_vertices the cells'
+ * vertices back to back. Dense assembly (assemble_into_memory, assemble_into_device_memory)
+ * supports these grids in FP64 (the regular rule of quadrilaterals is m x m Gauss: its
+ * point count must be a square); the other entry points refuse them. */
+GridT* grid_new_cells(const double* points, size_t npoints, const int64_t* cell_vertices, const uint8_t* cell_types,
+                      size_t ncells);
+/* Number of entries of the cells' vertex lists, the lists back to back, and the cell types. */
+size_t grid_cell_vertices_size(const GridT* grid);
+void grid_cell_vertices_copy(const GridT* grid, int64_t* cell_vertices);
+void grid_cell_types_copy(const GridT* grid, uint8_t* cell_types);
 /* Replace the vertex coordinates (same topology); drops cached device geometry. */
 int grid_set_points(GridT* grid, const double* points);
 
 /* The entry points the reference's python layer binds for grids, element families and
  * spaces (python/bempp/function_space.py:55-64,143-191), under the same names. */
 /* replaces single_element_grid_borrowed_create_bempp (src/bindings/ndgrid.rs:16-56): a grid
- * from the arrays of an ndgrid single-element grid. Flat triangles only (tdim 2, gdim 3,
- * geometry degree 1, 3 points per cell, entity_types[2] triangle); points f32 or f64
- * [npoints][3], cells [ncells][3] point indices. Identifier and upward-connectivity arrays
+ * from the arrays of an ndgrid single-element grid: triangles (3 points per cell,
+ * entity_types[2] triangle) or quadrilaterals (4 points per cell, entity_types[2]
+ * quadrilateral), tdim 2, gdim 3, geometry degree 1; points f32 or f64 [npoints][3], cells
+ * [ncells][points_per_cell] point indices. Identifier and upward-connectivity arrays
  * are not used; the cell-to-vertex topology (downward_connectivity[2][0], optional) must be a
  * relabelling of the cells' points. The arrays are copied. NULL on error. */
 GridT* single_element_grid_borrowed_create_bempp(size_t tdim, const size_t* id_sizes, const size_t* const* id_pointers,

@@ -1,0 +1,145 @@
+"""GPU dense assembly on grids with quadrilateral cells (cells.hip) against the oracle's
+general-cell assembly (tests/test_oracle_cells.py pins it): flat squares (cube), non-planar
+bilinear cells (cubed sphere), mixed triangle/quadrilateral grids (a cube with some squares
+split, the reference's mixed screen of tests/mixed_assembly.rs.bak:15-31); all four operators,
+Laplace and Helmholtz, P0 / P1 (Q1 on quadrilaterals) / DP1; relative Frobenius error
+<= 1e-12."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from bempp_amd.assembly.boundary import (BoundaryAssemblerOptions, OperatorType, Precision, create_helmholtz_assembler,
+                                         create_laplace_assembler)
+from bempp_amd.element import Continuity, Family, ReferenceCellType, create_family
+from bempp_amd.function_space import function_space
+from bempp_amd.grid import Grid
+from quad_meshes import cube, cubed_sphere, mixed_screen, split
+
+pytestmark = pytest.mark.gpu
+
+OPS = {"single_layer": OperatorType.SingleLayer, "double_layer": OperatorType.DoubleLayer,
+       "adjoint_double_layer": OperatorType.AdjointDoubleLayer, "hypersingular": OperatorType.Hypersingular}
+SPACES = {"dp0": (0, False), "p1": (1, True), "dp1": (1, False)}
+
+
+def _mesh(name):
+    if name == "cube":
+        pts, c = cube(2)
+        return pts, c
+    if name == "cubed_sphere":
+        return cubed_sphere(3)
+    if name == "mixed_cube":
+        pts, c = cube(2)
+        return pts, split(c, lambda i: i % 3 == 0)
+    return mixed_screen()[:2]
+
+
+def _options(nq_tri=6, nq_quad=16, nsing=4, mixed=False):
+    o = BoundaryAssemblerOptions()
+    o.set_regular_quadrature_degree(ReferenceCellType.Triangle, nq_tri)
+    o.set_regular_quadrature_degree(ReferenceCellType.Quadrilateral, nq_quad)
+    for t in [(ReferenceCellType.Triangle, ReferenceCellType.Triangle),
+              (ReferenceCellType.Quadrilateral, ReferenceCellType.Quadrilateral),
+              (ReferenceCellType.Triangle, ReferenceCellType.Quadrilateral),
+              (ReferenceCellType.Quadrilateral, ReferenceCellType.Triangle)]:
+        o.set_singular_quadrature_degree(*t, nsing)
+    if mixed:
+        o.set_precision(Precision.Mixed)
+    return o
+
+
+def _space(g, name):
+    deg, cont = SPACES[name]
+    return function_space(g, create_family(Family.Lagrange, deg, Continuity.Standard if cont else Continuity.Discontinuous))
+
+
+def _assembler(kind, op, k, o, dtype=None):
+    if kind == "laplace":
+        return create_laplace_assembler(OPS[op], o, dtype=dtype or np.float64)
+    return create_helmholtz_assembler(k, OPS[op], o, dtype=dtype or np.complex128)
+
+
+@pytest.mark.parametrize("mesh", ["cube", "cubed_sphere", "mixed_cube", "mixed_screen"])
+@pytest.mark.parametrize("kind,k", [("laplace", 0.0), ("helmholtz", 2.0)])
+@pytest.mark.parametrize("op", list(OPS))
+def test_quadrilateral_grids_match_oracle(mesh, kind, k, op):
+    pts, cells = _mesh(mesh)
+    g = Grid(pts, cells)
+    o = _options()
+    for sp in ["dp0", "p1", "dp1"]:
+        s = _space(g, sp)
+        print(f"[cells] {mesh} {kind} {op} {sp} ({len(cells)} cells)", flush=True)
+        A = _assembler(kind, op, k, o).assemble(s, s)
+        R = O.assemble_cells(kind, op, pts, cells, test=SPACES[sp], trial=SPACES[sp], k=k, nq_tri=6, nq_quad=16,
+                             nsing=4)
+        nr = np.linalg.norm(R)
+        if nr == 0.0:  # Laplace hypersingular on P0: identically zero
+            assert np.abs(A).max() == 0.0
+            continue
+        err = np.linalg.norm(A - R) / nr
+        print(f"[cells]   rel err {err:.2e}", flush=True)
+        assert err <= 1e-12
+
+
+@pytest.mark.parametrize("test,trial", [("dp0", "p1"), ("p1", "dp1")])
+def test_quadrilateral_mixed_spaces_and_other_rules(test, trial):
+    pts, cells = _mesh("mixed_cube")
+    g = Grid(pts, cells)
+    o = _options(nq_tri=7, nq_quad=9, nsing=5)
+    for kind, op, k in [("helmholtz", "double_layer", 3.0), ("laplace", "hypersingular", 0.0)]:
+        A = _assembler(kind, op, k, o).assemble(_space(g, trial), _space(g, test))
+        R = O.assemble_cells(kind, op, pts, cells, test=SPACES[test], trial=SPACES[trial], k=k, nq_tri=7, nq_quad=9,
+                             nsing=5)
+        if np.linalg.norm(R) == 0.0:
+            assert np.abs(A).max() == 0.0
+            continue
+        assert np.linalg.norm(A - R) / np.linalg.norm(R) <= 1e-12
+
+
+def test_quadrilateral_device_output_dtypes_and_modes():
+    """assemble_device overwrites, assemble_into_device_memory accumulates; float32 /
+    complex64 outputs; the mixed-precision option runs FP64 on these grids."""
+    import torch
+
+    pts, cells = _mesh("cubed_sphere")
+    g = Grid(pts, cells)
+    s = _space(g, "p1")
+    R = O.assemble_cells("helmholtz", "single_layer", pts, cells, test=(1, True), trial=(1, True), k=2.0)
+    a = _assembler("helmholtz", "single_layer", 2.0, _options())
+    D = a.assemble_device(s, s)
+    torch.cuda.synchronize()
+    assert np.linalg.norm(D.cpu().numpy() - R) / np.linalg.norm(R) <= 1e-12
+    a.assemble_into_device(s, s, D, accumulate=True)
+    torch.cuda.synchronize()
+    assert np.linalg.norm(D.cpu().numpy() - 2 * R) / np.linalg.norm(R) <= 1e-12
+    a32 = _assembler("helmholtz", "single_layer", 2.0, _options(), dtype=np.complex64)
+    A32 = a32.assemble(s, s)
+    assert A32.dtype == np.complex64 and np.linalg.norm(A32 - R) / np.linalg.norm(R) <= 1e-6
+    am = _assembler("helmholtz", "single_layer", 2.0, _options(mixed=True))
+    assert np.linalg.norm(am.assemble(s, s) - R) / np.linalg.norm(R) <= 1e-12
+    L = O.assemble_cells("laplace", "double_layer", pts, cells)
+    l32 = create_laplace_assembler(OperatorType.DoubleLayer, _options(), dtype=np.float32).assemble(
+        _space(g, "dp0"), _space(g, "dp0"))
+    assert l32.dtype == np.float32 and np.linalg.norm(l32 - L) / np.linalg.norm(L) <= 1e-6
+
+
+def test_quadrilateral_rule_must_be_square():
+    pts, cells = _mesh("cube")
+    g = Grid(pts, cells)
+    s = _space(g, "dp0")
+    a = _assembler("laplace", "single_layer", 0.0, BoundaryAssemblerOptions())  # default: 37 points
+    with pytest.raises(ValueError, match="square number"):
+        a.assemble(s, s)
+
+
+def test_larger_cubed_sphere():
+    """864 non-planar cells (12 x 12 per face): several trial chunks and thousands of
+    adjacent pairs, Helmholtz double layer on P1."""
+    pts, cells = cubed_sphere(12)
+    g = Grid(pts, cells)
+    s = _space(g, "p1")
+    A = _assembler("helmholtz", "double_layer", 4.0, _options(nq_quad=9)).assemble(s, s)
+    R = O.assemble_cells("helmholtz", "double_layer", pts, cells, test=(1, True), trial=(1, True), k=4.0, nq_quad=9)
+    err = np.linalg.norm(A - R) / np.linalg.norm(R)
+    print(f"[cells] cubed sphere 12: rel err {err:.2e}", flush=True)
+    assert err <= 1e-12
