@@ -89,6 +89,35 @@ def test_shape_gradient_fd_l3(kind, k):
     _check_gradient(kind, k, 3, seed=11, ndir=8)
 
 
+@pytest.mark.parametrize("kind,k", [("helmholtz", 4.0), ("laplace", 0.0)])
+def test_shape_gradient_every_component_l1(kind, k):
+    """Every component of the gradient (42 vertices x 3 coordinates of a perturbed level-1
+    icosphere) against central differences of the oracle's bilinear form, so that a wrong
+    vertex, coordinate or cell-side term shows in its own component."""
+    g0 = icosphere(1)
+    pts = S.perturb_radially(g0.points, 7)
+    cells = g0.cells
+    g = Grid(pts, cells)
+    s = function_space(g, create_family(Family.Lagrange, 0, Continuity.Discontinuous))
+    rng = np.random.default_rng(7)
+    if kind == "helmholtz":
+        u, v = S.complex_normal(len(cells), rng), S.complex_normal(len(cells), rng)
+    else:
+        u, v = rng.standard_normal(len(cells)), rng.standard_normal(len(cells))
+    G = _asm(kind, k).shape_gradient(s, s, u, v)
+    h = 1e-6
+    fd = np.zeros_like(G)
+    for vtx in range(len(pts)):
+        for c in range(3):
+            D = np.zeros_like(pts)
+            D[vtx, c] = h
+            fd[vtx, c] = (_oracle_form(kind, k, pts + D, cells, u, v) - _oracle_form(kind, k, pts - D, cells, u, v)) / (2 * h)
+    err = np.linalg.norm(G - fd) / np.linalg.norm(fd)
+    emax = np.abs(G - fd).max() / np.abs(fd).max()
+    print(f"[grad] {kind} L1 all {G.size} components: rel err {err:.2e}, max {emax:.2e}", flush=True)
+    assert err <= 1e-8 and emax <= 1e-7  # the differences themselves agree to ~6e-10 (h vs 2h)
+
+
 def test_shape_gradient_fd_l5_config():
     """The shape-gradient configuration: Helmholtz SL P0, k = 3, level-5 icosphere radially
     perturbed by 0.05 sum_{l<=4} a_lm Y_lm (seeded), u, v complex normal; 3 directions."""
