@@ -627,7 +627,7 @@ const int2* tile_list(int device, int nta, int ntb, bool symmetric, int* count) 
 
 // ---- kernel dispatch: tiled P0 x P0 or general, by operator, kernel and phase mode ----
 // CENTRED: phase tier (0: full reduction, 1: |k (r - R)| <= 0.5, 2: <= 0.25; tier 2
-// only for the P0 x P0 tile kernel, the general kernel runs tier 1 there).
+// for the tile kernels, the general kernel runs tier 1 there).
 // mode: 0 general kernel (atomics per cell pair), 1 P0 x P0 tiles, 2 tiles with LDS
 // accumulation (a P1 side; lds = dynamic LDS bytes).
 template <int OP, bool HELM, int CENTRED>
@@ -636,7 +636,7 @@ void launch_q(int mode, int Q, dim3 grid, hipStream_t s, const dev::RegularArgs&
   constexpr int GC = CENTRED > 1 ? 1 : CENTRED;
 #define BEMPP_LAUNCH(QQ)                                                                                \
   if (mode == 1) dev::regular_p0_tiled<QQ, OP, HELM, CENTRED><<<grid, 256, 0, s>>>(A, tiles, sym);       \
-  else if (mode == 2) dev::regular_tiled_lds<QQ, OP, HELM, GC><<<grid, 256, lds, s>>>(A, tiles, sym);    \
+  else if (mode == 2) dev::regular_tiled_lds<QQ, OP, HELM, CENTRED><<<grid, 256, lds, s>>>(A, tiles, sym); \
   else dev::regular_kernel<QQ, OP, HELM, GC><<<grid, 256, 0, s>>>(A);
   switch (kernel_q(Q)) {
     case 1: BEMPP_LAUNCH(1); break;
@@ -665,7 +665,7 @@ void launch_op(int op, int mode, int Q, dim3 grid, hipStream_t s, const dev::Reg
 void launch_regular(int op, bool helm, int tier, int mode, int Q, dim3 grid, hipStream_t s,
                     const dev::RegularArgs& A, const int2* tiles = nullptr, int sym = 0, size_t lds = 0) {
   if (!helm) launch_op<false, 0>(op, mode, Q, grid, s, A, tiles, sym, lds);
-  else if (tier == 2 && mode == 1) launch_op<true, 2>(op, mode, Q, grid, s, A, tiles, sym, lds);
+  else if (tier == 2 && (mode == 1 || mode == 2)) launch_op<true, 2>(op, mode, Q, grid, s, A, tiles, sym, lds);
   else if (tier >= 1) launch_op<true, 1>(op, mode, Q, grid, s, A, tiles, sym, lds);
   else launch_op<true, 0>(op, mode, Q, grid, s, A, tiles, sym, lds);
   HIP_CHECK(hipGetLastError());
@@ -857,6 +857,7 @@ Prepared prepare(const Assembler& a, const FunctionSpace& trial_c, const Functio
       const bool on = p < P.Q;
       R.wphi_test[p][b] = on ? (float)(w[p] * (test.nb == 3 ? R.phi[p][b] : (b == 0 ? 1.0 : 0.0))) : 0.0f;
       R.wphi_trial[p][b] = on ? (float)(w[p] * (trial.nb == 3 ? R.phi[p][b] : (b == 0 ? 1.0 : 0.0))) : 0.0f;
+      R.wphi_trial64[p][b] = on ? w[p] * (trial.nb == 3 ? R.phi[p][b] : (b == 0 ? 1.0 : 0.0)) : 0.0;
     }
   R.out = out;
   R.ld = (long long)test.size;

@@ -79,6 +79,9 @@ struct RegularArgs {
   // rule weight x basis value per side (a P0 side: {w, 0, 0}), FP32 (mixed-precision P1 tiles)
   float wphi_test[kMaxQ][3];
   float wphi_trial[kMaxQ][3];
+  // the trial side's in FP64 (the streamed FP64 P1 cell-pair sums apply the trial Jacobian once
+  // per pair)
+  double wphi_trial64[kMaxQ][3];
   // mixed precision, one grid: far_tiles[a * far_tiles_ld + b] != 0 when every cell pair of tile
   // (a, b) is far (then no pair of it is adjacent or near); null when not computed
   const unsigned char* far_tiles;
@@ -317,6 +320,30 @@ __device__ __forceinline__ cplx cmul(cplx a, cplx b) {
   return {fma(a.re, b.re, -a.im * b.im), fma(a.re, b.im, a.im * b.re)};
 }
 
+// Applies the trial Jacobian (and, CENTRED, the pair phase e^{ikR}) to the local block and
+// forms msum = sum_ab M_ab: with e scaled by J_y first, a centred pair pays two products.
+template <int CENTRED>
+__device__ __forceinline__ void scale_by_trial_jacobian(cplx (&M)[3][3], cplx& msum, cplx e, double jy) {
+  if constexpr (CENTRED) {
+    e.re *= jy;
+    e.im *= jy;
+  }
+  msum = {0.0, 0.0};
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      if constexpr (CENTRED) {
+        M[a][b] = cmul(M[a][b], e);
+      } else {
+        M[a][b].re *= jy;
+        M[a][b].im *= jy;
+      }
+      msum.re += M[a][b].re;
+      msum.im += M[a][b].im;
+    }
+}
+
 // Local block M_ab (without 1/(4 pi)) of one regular cell pair with a P1 side:
 // M_ab = sum_p sum_q K_pq (phi_a(p) w_p J_x)(psi_b(q) w_q J_y), summed over q first
 // into T_b(p); msum = sum_ab M_ab (for the hypersingular curl-curl term).
@@ -398,10 +425,9 @@ __device__ __forceinline__ void p1_local_streamed(Args& A, const double* tpts, c
       const cplx kv = kernel_scalar<OP, HELM, CENTRED>(x0 - sp[4 * q], x1 - sp[4 * q + 1], x2 - sp[4 * q + 2], A.k,
                                                        nx, ny, kR);
 #pragma unroll
-      for (int b = 0; b < 3; ++b) {
-        const double f = (A.nbs == 3 ? A.phi[q][b] : (b == 0 ? 1.0 : 0.0)) * sp[4 * q + 3];
-        T[b].re = fma(kv.re, f, T[b].re);
-        T[b].im = fma(kv.im, f, T[b].im);
+      for (int b = 0; b < 3; ++b) {  // trial weight x basis value; the trial Jacobian comes last
+        T[b].re = fma(kv.re, A.wphi_trial64[q][b], T[b].re);
+        T[b].im = fma(kv.im, A.wphi_trial64[q][b], T[b].im);
       }
     }
 #pragma unroll
@@ -414,15 +440,7 @@ __device__ __forceinline__ void p1_local_streamed(Args& A, const double* tpts, c
       }
     }
   }
-  msum = {0.0, 0.0};
-#pragma unroll
-  for (int a = 0; a < 3; ++a)
-#pragma unroll
-    for (int b = 0; b < 3; ++b) {
-      if constexpr (CENTRED) M[a][b] = cmul(M[a][b], e);
-      msum.re += M[a][b].re;
-      msum.im += M[a][b].im;
-    }
+  scale_by_trial_jacobian<CENTRED>(M, msum, e, sgeo[12]);
 }
 
 // p1_local for a rule of any point count (A.nq): test points streamed from tpts, trial
