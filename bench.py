@@ -234,6 +234,61 @@ def singular_csr_leg(level, k, steps):
     return out
 
 
+def quadrilateral_leg(m, k, steps):
+    """Dense assembly on a grid of quadrilaterals: the cube's faces cut into m x m squares and
+    projected onto the unit sphere (6 m^2 non-planar bilinear cells), Helmholtz SL P0, 4 x 4
+    Gauss points per cell, 4-point Duffy rules (the general-cell kernels, FP64, device output)."""
+    import numpy as np
+    import torch
+
+    from bempp_amd.assembly.boundary import BoundaryAssemblerOptions, OperatorType, create_helmholtz_assembler
+    from bempp_amd.element import Continuity, Family, ReferenceCellType, create_family
+    from bempp_amd.function_space import function_space
+    from bempp_amd.grid import Grid
+
+    g1 = np.linspace(-1.0, 1.0, m + 1)
+    pts, index, cells = [], {}, []
+    for axis in range(3):
+        for sign in (-1.0, 1.0):
+            a, b = (axis + 1) % 3, (axis + 2) % 3
+            if sign < 0:
+                a, b = b, a
+            for j in range(m):
+                for i in range(m):
+                    quad = []
+                    for dj, di in ((0, 0), (0, 1), (1, 0), (1, 1)):
+                        p = [0.0, 0.0, 0.0]
+                        p[axis], p[a], p[b] = sign, g1[i + di], g1[j + dj]
+                        key = tuple(round(x, 12) for x in p)
+                        if key not in index:
+                            index[key] = len(pts)
+                            pts.append(p)
+                        quad.append(index[key])
+                    cells.append(quad)
+    pts = np.array(pts)
+    pts /= np.linalg.norm(pts, axis=1)[:, None]
+    grid = Grid(pts, np.array(cells))
+    space = function_space(grid, create_family(Family.Lagrange, 0, Continuity.Discontinuous))
+    o = BoundaryAssemblerOptions()
+    o.set_regular_quadrature_degree(ReferenceCellType.Quadrilateral, 16)
+    o.set_singular_quadrature_degree(ReferenceCellType.Quadrilateral, ReferenceCellType.Quadrilateral, 4)
+    a = create_helmholtz_assembler(k, OperatorType.SingleLayer, o)
+    n = space.global_size
+    out = torch.empty((n, n), dtype=torch.complex128, device="cuda").t()
+    a.assemble_into_device(space, space, out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        a.assemble_into_device(space, space, out)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    print(f"[bench] quadrilateral grid ({n} cells): {ms:.2f} ms", file=sys.stderr, flush=True)
+    return {"workload": f"Helmholtz SL P0, k = {k:g}, cubed sphere of {n} non-planar quadrilaterals "
+                        f"({m} x {m} per face), 4 x 4 Gauss points, 4-point Duffy rules, complex128 device output",
+            "cells": n, "ms_per_assembly": ms, "entries_per_s": n * n / (ms * 1e-3),
+            "point_pairs_per_s": n * n * 256 / (ms * 1e-3)}
+
+
 def copy_calibration():
     """A plain device copy of 1 GiB (read 1 GiB + write 1 GiB per dispatch), three times:
     the reference point that scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -483,6 +538,7 @@ def main():
     # inputs through the entry point that allocates nothing
     gradient = shape_gradient_leg(args.level, args.warmup, args.steps)
     singular_csr = singular_csr_leg(args.level, args.k, max(3, args.steps))
+    quadrilaterals = quadrilateral_leg(40, args.k, 3)
 
     # the oracle spot check of the sample rows (first 64 rows: test cells 0..63 against all trial cells)
     ref = O.assemble("helmholtz", "single_layer", grid.points, grid.cells, k=args.k, fast=False, rows=(0, rows))
@@ -568,6 +624,7 @@ def main():
         "batched": batched,
         "shape_gradient": gradient,
         "singular_csr": singular_csr,
+        "quadrilateral_grid": quadrilaterals,
     }
     print(json.dumps(line))
 
