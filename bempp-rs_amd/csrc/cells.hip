@@ -18,6 +18,8 @@
 // with atomics.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <map>
 #include <string>
 #include <tuple>
@@ -106,16 +108,19 @@ __device__ __forceinline__ void cell_point(const double* V, bool quad, double x,
 
 // adds the integrand of one point pair (without 1/(4 pi)) to M:
 // wpq = (w_p J_p)(w_q J_q), ww = w_p w_q (the curl-curl term's weight, its Jacobians cancel)
-template <int OP, bool HELM, int NBT, int NBS, typename PP, typename PC, typename QN, typename QP, typename QC>
+// TIER > 0: the phase relative to kR (the caller applies e^{ikR} once per cell pair).
+template <int OP, bool HELM, int NBT, int NBS, int TIER = 0, typename PP, typename PC, typename QN, typename QP,
+          typename QC>
 __device__ __forceinline__ void add_point_pair(const double* xp, const double* np, const PP& php, const PC& cp,
                                                const double* xq, const QN& nq, const QP& phq, const QC& cq,
-                                               double wpq, double ww, double k, cplx (&M)[NBT][NBS]) {
+                                               double wpq, double ww, double k, cplx (&M)[NBT][NBS],
+                                               double kR = 0.0) {
   const double dx = xp[0] - xq[0], dy = xp[1] - xq[1], dz = xp[2] - xq[2];
   const double r2 = dx * dx + dy * dy + dz * dz;
   double dn = 0.0;
   if constexpr (OP == 1) dn = dx * nq[0] + dy * nq[1] + dz * nq[2];
   if constexpr (OP == 2) dn = -(dx * np[0] + dy * np[1] + dz * np[2]);
-  const cplx g = kernel_scalar_r2<OP, HELM, 0>(r2, dn, k);
+  const cplx g = kernel_scalar_r2<OP, HELM, TIER>(r2, dn, k, kR);
   if constexpr (OP != 3) {
 #pragma unroll
     for (int a = 0; a < NBT; ++a)
@@ -139,7 +144,18 @@ __device__ __forceinline__ void add_point_pair(const double* xp, const double* n
   }
 }
 
-template <int OP, bool HELM, int NBT, int NBS>
+// centroid of a cell from its vertex coordinates (the bilinear centre x(1/2, 1/2) of a
+// quadrilateral is its vertex mean too)
+template <typename V>
+__device__ __forceinline__ void cell_centroid(const V& cv, bool quad, double* c) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    c[d] = quad ? 0.25 * (cv[d] + cv[3 + d] + cv[6 + d] + cv[9 + d]) : (cv[d] + cv[3 + d] + cv[6 + d]) * (1.0 / 3.0);
+}
+
+// TIER (Helmholtz): 0 full argument reduction per point pair; 1, 2 the phase relative to the
+// centroid distance R, |k (r - R)| <= 0.5 / 0.25 over the whole grid pair.
+template <int OP, bool HELM, int NBT, int NBS, int TIER>
 __global__ void __launch_bounds__(256) cells_regular_kernel(const CellsArgs A) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool i_ok = i < A.n_test;
@@ -147,6 +163,9 @@ __global__ void __launch_bounds__(256) cells_regular_kernel(const CellsArgs A) {
   int tv[4], td[NBT];
 #pragma unroll
   for (int m = 0; m < 4; ++m) tv[m] = A.tvid[4 * il + m];
+  double ct[3];
+  if constexpr (TIER > 0) cell_centroid(A.tcv + 12 * (size_t)il, tv[3] >= 0, ct);
+  const auto scv = constant_view(A.scv);
 #pragma unroll
   for (int a = 0; a < NBT; ++a) td[a] = A.tdofs[4 * il + a];
   const int p0 = A.tfirst[il], p1 = A.tfirst[il + 1];
@@ -171,6 +190,13 @@ __global__ void __launch_bounds__(256) cells_regular_kernel(const CellsArgs A) {
     for (int a = 0; a < NBT; ++a)
 #pragma unroll
       for (int b = 0; b < NBS; ++b) M[a][b] = {0.0, 0.0};
+    double kR = 0.0;
+    cplx e = {1.0, 0.0};
+    if constexpr (TIER > 0) {
+      double cs[3];
+      cell_centroid(scv + 12 * (size_t)j, svid[4 * j + 3] >= 0, cs);
+      e = pair_phase(ct, cs, A.k, &kR);
+    }
     const int q0 = sfirst[j], q1 = sfirst[j + 1];
     for (int p = p0; p < p1; ++p) {
       const double* P = A.trec + (size_t)p * kRec;
@@ -195,9 +221,14 @@ __global__ void __launch_bounds__(256) cells_regular_kernel(const CellsArgs A) {
 #pragma unroll
             for (int d = 0; d < 3; ++d) cq[b][d] = Q[12 + 3 * b + d];
         }
-        add_point_pair<OP, HELM, NBT, NBS>(xp, np, php, cp, xq, nq, phq, cq, wJp * Q[3], wp * Q[7], A.k, M);
+        add_point_pair<OP, HELM, NBT, NBS, TIER>(xp, np, php, cp, xq, nq, phq, cq, wJp * Q[3], wp * Q[7], A.k, M, kR);
       }
     }
+    if constexpr (TIER > 0)
+#pragma unroll
+      for (int a = 0; a < NBT; ++a)
+#pragma unroll
+        for (int b = 0; b < NBS; ++b) M[a][b] = cmul(M[a][b], e);
 #pragma unroll
     for (int b = 0; b < NBS; ++b) {
       const int col = sdofs[4 * j + b];
@@ -394,28 +425,51 @@ void adjacent_cell_pairs(const Grid& g, const Options& o, std::vector<int4>& pai
 }
 
 template <int OP, bool HELM, int NBT, int NBS>
-void launch_cells(dim3 rg, dim3 sg, hipStream_t s, const dev::CellsArgs& A) {
-  dev::cells_regular_kernel<OP, HELM, NBT, NBS><<<rg, 256, 0, s>>>(A);
+void launch_cells(int tier, dim3 rg, dim3 sg, hipStream_t s, const dev::CellsArgs& A) {
+  if (HELM && tier == 2) dev::cells_regular_kernel<OP, HELM, NBT, NBS, 2><<<rg, 256, 0, s>>>(A);
+  else if (HELM && tier == 1) dev::cells_regular_kernel<OP, HELM, NBT, NBS, 1><<<rg, 256, 0, s>>>(A);
+  else dev::cells_regular_kernel<OP, HELM, NBT, NBS, 0><<<rg, 256, 0, s>>>(A);
   if (A.npairs > 0) dev::cells_singular_kernel<OP, HELM, NBT, NBS><<<sg, 256, 0, s>>>(A);
 }
 
 template <int OP, bool HELM>
-void launch_cells_nb(int nbt, int nbs, dim3 rg, dim3 sg, hipStream_t s, const dev::CellsArgs& A) {
-  if (nbt == 1 && nbs == 1) launch_cells<OP, HELM, 1, 1>(rg, sg, s, A);
-  else if (nbt == 1) launch_cells<OP, HELM, 1, 4>(rg, sg, s, A);
-  else if (nbs == 1) launch_cells<OP, HELM, 4, 1>(rg, sg, s, A);
-  else launch_cells<OP, HELM, 4, 4>(rg, sg, s, A);
+void launch_cells_nb(int nbt, int nbs, int tier, dim3 rg, dim3 sg, hipStream_t s, const dev::CellsArgs& A) {
+  if (nbt == 1 && nbs == 1) launch_cells<OP, HELM, 1, 1>(tier, rg, sg, s, A);
+  else if (nbt == 1) launch_cells<OP, HELM, 1, 4>(tier, rg, sg, s, A);
+  else if (nbs == 1) launch_cells<OP, HELM, 4, 1>(tier, rg, sg, s, A);
+  else launch_cells<OP, HELM, 4, 4>(tier, rg, sg, s, A);
 }
 
 template <bool HELM>
-void launch_cells_op(int op, int nbt, int nbs, dim3 rg, dim3 sg, hipStream_t s, const dev::CellsArgs& A) {
+void launch_cells_op(int op, int nbt, int nbs, int tier, dim3 rg, dim3 sg, hipStream_t s, const dev::CellsArgs& A) {
   switch (op) {
-    case kSL: launch_cells_nb<0, HELM>(nbt, nbs, rg, sg, s, A); break;
-    case kDL: launch_cells_nb<1, HELM>(nbt, nbs, rg, sg, s, A); break;
-    case kADL: launch_cells_nb<2, HELM>(nbt, nbs, rg, sg, s, A); break;
-    case kHS: launch_cells_nb<3, HELM>(nbt, nbs, rg, sg, s, A); break;
+    case kSL: launch_cells_nb<0, HELM>(nbt, nbs, tier, rg, sg, s, A); break;
+    case kDL: launch_cells_nb<1, HELM>(nbt, nbs, tier, rg, sg, s, A); break;
+    case kADL: launch_cells_nb<2, HELM>(nbt, nbs, tier, rg, sg, s, A); break;
+    case kHS: launch_cells_nb<3, HELM>(nbt, nbs, tier, rg, sg, s, A); break;
     default: throw Error("operator not supported");
   }
+}
+
+// largest centroid-to-vertex distance over a grid's cells: every point of a cell lies within
+// it of the centroid (a bilinear cell lies in the convex hull of its vertices)
+double max_cell_radius(const Grid& g) {
+  double rmax = 0.0;
+  for (int64_t c = 0; c < g.nc(); ++c) {
+    const int n = g.cell_nv(c);
+    double cen[3] = {0.0, 0.0, 0.0};
+    for (int m = 0; m < n; ++m)
+      for (int d = 0; d < 3; ++d) cen[d] += g.points[3 * g.vertex(c, m) + d] / n;
+    for (int m = 0; m < n; ++m) {
+      double r2 = 0.0;
+      for (int d = 0; d < 3; ++d) {
+        const double x = g.points[3 * g.vertex(c, m) + d] - cen[d];
+        r2 += x * x;
+      }
+      rmax = std::max(rmax, std::sqrt(r2));
+    }
+  }
+  return rmax;
 }
 
 }  // namespace
@@ -472,10 +526,17 @@ void assemble_cells_device(const Assembler& a, const FunctionSpace& trial, const
   const dim3 rg(bx, (unsigned)((A.n_trial + chunk - 1) / chunk));
   const dim3 sg((unsigned)((A.npairs + 3) / 4));
   const int nbt = test.degree == 0 ? 1 : 4, nbs = trial.degree == 0 ? 1 : 4;
+  // phase tier from the grid-wide bound |k (r - R)| <= |k| (rad_test + rad_trial), with a
+  // margin for rounding (the tiers' polynomials hold to 0.5 and 0.25 exactly)
+  int tier = 0;
+  if (a.kind == kHelmholtz && a.k != 0.0) {
+    const double bound = std::fabs(a.k) * (max_cell_radius(*test.grid) + max_cell_radius(*trial.grid)) * (1.0 + 1e-12);
+    tier = bound <= 0.25 ? 2 : (bound <= 0.5 ? 1 : 0);
+  }
   if (a.kind == kHelmholtz)
-    launch_cells_op<true>(a.op, nbt, nbs, rg, sg, s, A);
+    launch_cells_op<true>(a.op, nbt, nbs, tier, rg, sg, s, A);
   else
-    launch_cells_op<false>(a.op, nbt, nbs, rg, sg, s, A);
+    launch_cells_op<false>(a.op, nbt, nbs, 0, rg, sg, s, A);
   HIP_CHECK_Q(hipGetLastError());
   // the uploads above are freed when `keep` goes out of scope: wait for the kernels
   HIP_CHECK_Q(hipStreamSynchronize(s));

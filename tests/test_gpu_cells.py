@@ -123,6 +123,26 @@ def test_quadrilateral_device_output_dtypes_and_modes():
     assert l32.dtype == np.float32 and np.linalg.norm(l32 - L) / np.linalg.norm(L) <= 1e-6
 
 
+@pytest.mark.parametrize("k,tier", [(1.2, 1), (0.6, 2)])
+@pytest.mark.parametrize("op", list(OPS))
+def test_quadrilateral_centred_phase_tiers(k, tier, op):
+    """Small |k| x cell size on a 384-cell cubed sphere: the regular kernel takes the phase
+    relative to each cell pair's centroid distance (tier 1: |k| (rad_test + rad_trial) <= 0.5,
+    tier 2: <= 0.25, rad the largest centroid-to-vertex distance)."""
+    pts, cells = cubed_sphere(8)
+    rad = max(np.linalg.norm(pts[c] - pts[c].mean(axis=0), axis=1).max() for c in cells)
+    bound = abs(k) * 2 * rad
+    assert (bound <= 0.25) == (tier == 2) and bound <= 0.5
+    g = Grid(pts, cells)
+    for sp in ["dp0", "p1"]:
+        s = _space(g, sp)
+        A = _assembler("helmholtz", op, k, _options(nq_quad=9)).assemble(s, s)
+        R = O.assemble_cells("helmholtz", op, pts, cells, test=SPACES[sp], trial=SPACES[sp], k=k, nq_quad=9)
+        err = np.linalg.norm(A - R) / np.linalg.norm(R)
+        print(f"[cells] tier {tier} k={k} {op} {sp}: rel err {err:.2e}", flush=True)
+        assert err <= 1e-12
+
+
 def test_quadrilateral_rule_must_be_square():
     pts, cells = _mesh("cube")
     g = Grid(pts, cells)
