@@ -1,5 +1,6 @@
-// cells.hip -- dense assembly on grids with quadrilateral cells (single-element quadrilateral
-// grids and mixed triangle/quadrilateral grids of degree-1 geometry), FP64.
+// cells.hip -- dense assembly, and the sparse singular part and correction, on grids with
+// quadrilateral cells (single-element quadrilateral grids and mixed triangle/quadrilateral
+// grids of degree-1 geometry), FP64.
 //
 // The reference's drivers take any cell type (src/boundary_assemblers.rs:396-521 loops over
 // the grid's entity_types(2), :524-562 picks a Duffy rule per (test type, trial type)).
@@ -59,6 +60,8 @@ struct CellsArgs {
   const double* rule;  // adjacent-pair rules, records (test xi, eta, trial xi, eta, weight)
   const int4* pairs;   // test cell, trial cell, first rule record, record count
   int npairs;
+  int correction;      // adjacent pairs under the cells' regular rules instead of their pair rules
+  double* coo;         // non-null: the adjacent pairs' values in COO layout instead of `out`
 };
 
 // Point, normal, Jacobian, basis values and J-scaled curls of a cell at (x, y); V: the
@@ -261,12 +264,42 @@ __global__ void __launch_bounds__(256) cells_singular_kernel(const CellsArgs A) 
   for (int a = 0; a < NBT; ++a)
 #pragma unroll
     for (int b = 0; b < NBS; ++b) M[a][b] = {0.0, 0.0};
-  for (int i = lane; i < pr.w; i += 64) {
-    const double* r = A.rule + 5 * (size_t)(pr.z + i);
-    double xp[3], np[3], Jp, php[4], cp[4][3], xq[3], nq[3], Jq, phq[4], cq[4][3];
-    cell_point<NBT>(tV, tq, r[0], r[1], xp, np, Jp, php, cp);
-    cell_point<NBS>(sV, sq, r[2], r[3], xq, nq, Jq, phq, cq);
-    add_point_pair<OP, HELM, NBT, NBS>(xp, np, php, cp, xq, nq, phq, cq, r[4] * Jp * Jq, r[4], A.k, M);
+  if (A.correction) {
+    // the regular rule of each cell: every (test point, trial point) pair of their records
+    const int p0 = A.tfirst[tc], q0 = A.sfirst[sc], nqs = A.sfirst[sc + 1] - q0;
+    const int n = (A.tfirst[tc + 1] - p0) * nqs;
+    for (int i = lane; i < n; i += 64) {
+      const int ip = i / nqs;
+      const double* P = A.trec + (size_t)(p0 + ip) * kRec;
+      const double* Q = A.srec + (size_t)(q0 + i - ip * nqs) * kRec;
+      const double xp[3] = {P[0], P[1], P[2]}, np[3] = {P[4], P[5], P[6]};
+      const double xq[3] = {Q[0], Q[1], Q[2]}, nq[3] = {Q[4], Q[5], Q[6]};
+      // coincident points (a cell with itself) are left out, as a point-to-point fast
+      // multipole evaluation does
+      if (xp[0] == xq[0] && xp[1] == xq[1] && xp[2] == xq[2]) continue;
+      double php[NBT], cp[NBT][3], phq[NBS], cq[NBS][3];
+#pragma unroll
+      for (int a = 0; a < NBT; ++a) {
+        php[a] = P[8 + a];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) cp[a][d] = P[12 + 3 * a + d];
+      }
+#pragma unroll
+      for (int b = 0; b < NBS; ++b) {
+        phq[b] = Q[8 + b];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) cq[b][d] = Q[12 + 3 * b + d];
+      }
+      add_point_pair<OP, HELM, NBT, NBS>(xp, np, php, cp, xq, nq, phq, cq, P[3] * Q[3], P[7] * Q[7], A.k, M);
+    }
+  } else {
+    for (int i = lane; i < pr.w; i += 64) {
+      const double* r = A.rule + 5 * (size_t)(pr.z + i);
+      double xp[3], np[3], Jp, php[4], cp[4][3], xq[3], nq[3], Jq, phq[4], cq[4][3];
+      cell_point<NBT>(tV, tq, r[0], r[1], xp, np, Jp, php, cp);
+      cell_point<NBS>(sV, sq, r[2], r[3], xq, nq, Jq, phq, cq);
+      add_point_pair<OP, HELM, NBT, NBS>(xp, np, php, cp, xq, nq, phq, cq, r[4] * Jp * Jq, r[4], A.k, M);
+    }
   }
 #pragma unroll
   for (int a = 0; a < NBT; ++a)
@@ -276,6 +309,19 @@ __global__ void __launch_bounds__(256) cells_singular_kernel(const CellsArgs A) 
       if constexpr (HELM) M[a][b].im = wave_sum(M[a][b].im);
     }
   if (lane != 0) return;
+  if (A.coo) {
+    // position pair * NBT * NBS + a + NBT * b, (re, im); entries of basis functions a cell
+    // does not have (a triangle's fourth) are written as zeros and dropped by their DOF -1
+    double* o = A.coo + 2 * (size_t)pair * NBT * NBS;
+#pragma unroll
+    for (int b = 0; b < NBS; ++b)
+#pragma unroll
+      for (int a = 0; a < NBT; ++a) {
+        o[2 * (a + NBT * b)] = M[a][b].re * kInv4Pi;
+        o[2 * (a + NBT * b) + 1] = HELM ? M[a][b].im * kInv4Pi : 0.0;
+      }
+    return;
+  }
 #pragma unroll
   for (int b = 0; b < NBS; ++b) {
     const int col = A.sdofs[4 * sc + b];
@@ -424,11 +470,12 @@ void adjacent_cell_pairs(const Grid& g, const Options& o, std::vector<int4>& pai
     }
 }
 
+// tier < 0: the adjacent pairs only (no regular kernel)
 template <int OP, bool HELM, int NBT, int NBS>
 void launch_cells(int tier, dim3 rg, dim3 sg, hipStream_t s, const dev::CellsArgs& A) {
   if (HELM && tier == 2) dev::cells_regular_kernel<OP, HELM, NBT, NBS, 2><<<rg, 256, 0, s>>>(A);
   else if (HELM && tier == 1) dev::cells_regular_kernel<OP, HELM, NBT, NBS, 1><<<rg, 256, 0, s>>>(A);
-  else dev::cells_regular_kernel<OP, HELM, NBT, NBS, 0><<<rg, 256, 0, s>>>(A);
+  else if (tier >= 0) dev::cells_regular_kernel<OP, HELM, NBT, NBS, 0><<<rg, 256, 0, s>>>(A);
   if (A.npairs > 0) dev::cells_singular_kernel<OP, HELM, NBT, NBS><<<sg, 256, 0, s>>>(A);
 }
 
@@ -472,20 +519,10 @@ double max_cell_radius(const Grid& g) {
   return rmax;
 }
 
-}  // namespace
-
-void assemble_cells_device(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test, void* dev_out,
-                           bool accumulate, void* stream) {
-  const hipStream_t s = (hipStream_t)stream;
-  const long long n_out = (long long)test.size * (long long)trial.size * (is_complex(a.dtype) ? 2 : 1);
-  if (!accumulate) {
-    if (a.dtype == kF32 || a.dtype == kC32)
-      dev::fill_zero_kernel<<<2048, 256, 0, s>>>((float*)dev_out, n_out);
-    else
-      dev::fill_zero_kernel<<<2048, 256, 0, s>>>((double*)dev_out, n_out);
-    HIP_CHECK_Q(hipGetLastError());
-  }
-  if (test.grid->nc() == 0 || trial.grid->nc() == 0) return;
+// The cells' point records, vertex and DOF tables and (same grid) adjacent pairs with their
+// rules, uploaded into `keep`; `A` points at them.
+void upload_cells(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test, hipStream_t s,
+                  std::vector<DeviceBuf>& keep, dev::CellsArgs& A) {
   std::vector<double> trec, srec, tcv, scv, rule;
   std::vector<int> tfirst, sfirst, tvid, svid, tdofs, sdofs;
   std::vector<int4> pairs;
@@ -495,9 +532,8 @@ void assemble_cells_device(const Assembler& a, const FunctionSpace& trial, const
   cell_tables(trial, svid, sdofs, scv);
   const bool same = test.grid == trial.grid;
   if (same) adjacent_cell_pairs(*test.grid, a.options, pairs, rule);
-  std::vector<DeviceBuf> keep;
   keep.reserve(16);
-  dev::CellsArgs A{};
+  A = dev::CellsArgs{};
   A.trec = upload_to(keep, trec, s);
   A.srec = upload_to(keep, srec, s);
   A.tfirst = upload_to(keep, tfirst, s);
@@ -515,9 +551,58 @@ void assemble_cells_device(const Assembler& a, const FunctionSpace& trial, const
   A.n_trial = (int)trial.grid->nc();
   A.same_grid = same ? 1 : 0;
   A.k = a.k;
-  A.out = dev_out;
   A.ld = (long long)test.size;
   A.out_type = cells_out_type(a.dtype);
+}
+
+}  // namespace
+
+void coo_to_csr_device(const double* vals, const int4* pairs, int npairs, const int* test_dofs, int nbt, int tstride,
+                       const int* trial_dofs, int nbs, int sstride, int64_t nrows, int64_t ncols, DType dtype,
+                       hipStream_t s, CsrHost& out);  // csr.hip
+
+void assemble_cells_singular_csr(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test,
+                                 bool correction, void* stream, CsrHost& out) {
+  const hipStream_t s = (hipStream_t)stream;
+  out.indptr.assign((size_t)test.size + 1, 0);
+  out.indices.clear();
+  out.data.clear();
+  if (test.grid != trial.grid || test.grid->nc() == 0) return;  // no neighbouring cells across grids
+  std::vector<DeviceBuf> keep;
+  dev::CellsArgs A;
+  upload_cells(a, trial, test, s, keep, A);
+  if (A.npairs == 0) return;
+  const int nbt = test.degree == 0 ? 1 : 4, nbs = trial.degree == 0 ? 1 : 4;
+  keep.emplace_back();
+  HIP_CHECK_Q(hipMalloc(&keep.back().p, (size_t)A.npairs * nbt * nbs * 2 * sizeof(double)));
+  A.coo = (double*)keep.back().p;
+  A.correction = correction ? 1 : 0;
+  const dim3 sg((unsigned)((A.npairs + 3) / 4));
+  if (a.kind == kHelmholtz)
+    launch_cells_op<true>(a.op, nbt, nbs, -1, dim3(1), sg, s, A);
+  else
+    launch_cells_op<false>(a.op, nbt, nbs, -1, dim3(1), sg, s, A);
+  HIP_CHECK_Q(hipGetLastError());
+  coo_to_csr_device(A.coo, A.pairs, A.npairs, A.tdofs, nbt, 4, A.sdofs, nbs, 4, test.size, trial.size, a.dtype, s, out);
+  // coo_to_csr_device ends with a stream synchronisation: the uploads may be freed
+}
+
+void assemble_cells_device(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test, void* dev_out,
+                           bool accumulate, void* stream) {
+  const hipStream_t s = (hipStream_t)stream;
+  const long long n_out = (long long)test.size * (long long)trial.size * (is_complex(a.dtype) ? 2 : 1);
+  if (!accumulate) {
+    if (a.dtype == kF32 || a.dtype == kC32)
+      dev::fill_zero_kernel<<<2048, 256, 0, s>>>((float*)dev_out, n_out);
+    else
+      dev::fill_zero_kernel<<<2048, 256, 0, s>>>((double*)dev_out, n_out);
+    HIP_CHECK_Q(hipGetLastError());
+  }
+  if (test.grid->nc() == 0 || trial.grid->nc() == 0) return;
+  std::vector<DeviceBuf> keep;
+  dev::CellsArgs A;
+  upload_cells(a, trial, test, s, keep, A);
+  A.out = dev_out;
   const int bx = (A.n_test + 255) / 256;
   long long chunk = ((long long)bx * A.n_trial + 16383) / 16384;
   chunk = std::max(4LL, std::min(64LL, chunk));

@@ -183,6 +183,9 @@ struct CsrHost {
 // no stored entries when no |entry| exceeds 1e-10.
 void assemble_singular_csr(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test, bool correction,
                            CsrHost& out);
+// The same on a grid with quadrilateral cells, on `stream`.
+void assemble_cells_singular_csr(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test,
+                                 bool correction, void* stream, CsrHost& out);
 // Potential operator [npoints x ndofs] (column-major) of a single/double-layer assembler
 // on `space` at the given points; device variant (points and output in device memory,
 // on `stream`) and host variant (accumulates into a host buffer).

@@ -35,11 +35,14 @@ namespace {
 using u64 = unsigned long long;
 
 // COO position e = pair * nbt * nbs + t + nbt * b (t: test basis function, b: trial basis
-// function), the layout of the adjacent-pair kernels' COO mode.
+// function), the layout of the adjacent-pair kernels' COO mode. DOF tables: `tstride` /
+// `sstride` entries per cell, -1 for a basis function the cell does not have (a triangle's
+// fourth on a grid with quadrilaterals); such entries take the key nrows * ncols, which
+// sorts after every real one.
 __global__ void __launch_bounds__(256) csr_keys_kernel(const int4* __restrict__ pairs, long long n, int nbt, int nbs,
-                                                       const int* __restrict__ test_dofs,
-                                                       const int* __restrict__ trial_dofs, u64 ncols,
-                                                       const double* __restrict__ vals, int dtype,
+                                                       const int* __restrict__ test_dofs, int tstride,
+                                                       const int* __restrict__ trial_dofs, int sstride, u64 nrows,
+                                                       u64 ncols, const double* __restrict__ vals, int dtype,
                                                        u64* __restrict__ keys, unsigned* __restrict__ pos,
                                                        int* __restrict__ any_nonzero) {
   const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -50,11 +53,12 @@ __global__ void __launch_bounds__(256) csr_keys_kernel(const int4* __restrict__ 
     const int r = (int)(e - p * nbl);
     const int t = r % nbt, b = r / nbt;
     const int4 pr = pairs[p];
-    const u64 row = (u64)test_dofs[(long long)pr.x * nbt + t];
-    const u64 col = (u64)trial_dofs[(long long)pr.y * nbs + b];
-    keys[e] = row * ncols + col;
+    const int row = test_dofs[(long long)pr.x * tstride + t];
+    const int col = trial_dofs[(long long)pr.y * sstride + b];
+    const bool valid = row >= 0 && col >= 0;
+    keys[e] = valid ? (u64)row * ncols + (u64)col : nrows * ncols;
     pos[e] = (unsigned)e;
-    const double re = vals[2 * e], im = vals[2 * e + 1];
+    const double re = valid ? vals[2 * e] : 0.0, im = valid ? vals[2 * e + 1] : 0.0;
     double mag;
     switch (dtype) {
       case kF64: mag = fabs(re); break;
@@ -148,16 +152,16 @@ void launch_segments(dim3 g, hipStream_t s, const u64* keys, const unsigned* ord
 
 }  // namespace
 
-void coo_to_csr_device(const double* vals, const int4* pairs, int npairs, const int* test_dofs, int nbt,
-                       const int* trial_dofs, int nbs, int64_t nrows, int64_t ncols, DType dtype, hipStream_t s,
-                       CsrHost& out) {
+void coo_to_csr_device(const double* vals, const int4* pairs, int npairs, const int* test_dofs, int nbt, int tstride,
+                       const int* trial_dofs, int nbs, int sstride, int64_t nrows, int64_t ncols, DType dtype,
+                       hipStream_t s, CsrHost& out) {
   out.indptr.assign((size_t)nrows + 1, 0);
   out.indices.clear();
   out.data.clear();
   const long long n = (long long)npairs * nbt * nbs;
   if (n == 0 || nrows == 0 || ncols == 0) return;
   if (n >= (1LL << 32)) throw Error("too many singular COO entries for the device CSR build (2^32)");
-  const u64 maxkey = (u64)nrows * (u64)ncols - 1;
+  const u64 maxkey = (u64)nrows * (u64)ncols;  // the key of absent entries
   unsigned bits = 1;
   while (bits < 64 && (maxkey >> bits) != 0) ++bits;
   const size_t vb = dtype_size(dtype);
@@ -190,8 +194,8 @@ void coo_to_csr_device(const double* vals, const int4* pairs, int npairs, const 
 
   const dim3 g((unsigned)((n + 255) / 256));
   dev::fill_zero_kernel<int><<<1, 64, 0, s>>>(flag, 1);
-  csr_keys_kernel<<<g, 256, 0, s>>>(pairs, n, nbt, nbs, test_dofs, trial_dofs, (u64)ncols, vals, (int)dtype, keys,
-                                    ord, flag);
+  csr_keys_kernel<<<g, 256, 0, s>>>(pairs, n, nbt, nbs, test_dofs, tstride, trial_dofs, sstride, (u64)nrows,
+                                    (u64)ncols, vals, (int)dtype, keys, ord, flag);
   HIP_CHECK_C(hipGetLastError());
   size_t b = sort_tmp;
   HIP_CHECK_C(rocprim::radix_sort_pairs(tmp, b, (const u64*)keys, keys_sorted, (const unsigned*)ord, ord_sorted,
@@ -209,10 +213,13 @@ void coo_to_csr_device(const double* vals, const int4* pairs, int npairs, const 
   HIP_CHECK_C(hipGetLastError());
   int any = 0;
   unsigned nnz = 0;
+  u64 last = 0;
   HIP_CHECK_C(hipMemcpyAsync(&any, flag, sizeof(int), hipMemcpyDeviceToHost, s));
   HIP_CHECK_C(hipMemcpyAsync(&nnz, scan + (n - 1), sizeof(unsigned), hipMemcpyDeviceToHost, s));
+  HIP_CHECK_C(hipMemcpyAsync(&last, keys_sorted + (n - 1), sizeof(u64), hipMemcpyDeviceToHost, s));
   HIP_CHECK_C(hipStreamSynchronize(s));
   if (!any) return;
+  if (last == maxkey) --nnz;  // the run of absent entries, last in key order
   csr_indptr_kernel<<<dim3((unsigned)((nnz + 1 + 255) / 256)), 256, 0, s>>>(urow, (long long)nnz, (long long)nrows,
                                                                             indptr);
   HIP_CHECK_C(hipGetLastError());

@@ -1159,12 +1159,17 @@ void assemble_singular_correction_coo(const Assembler& a, const FunctionSpace& t
   adjacent_coo(a, trial, test, true, rows, cols, vals);
 }
 
-void coo_to_csr_device(const double* vals, const int4* pairs, int npairs, const int* test_dofs, int nbt,
-                       const int* trial_dofs, int nbs, int64_t nrows, int64_t ncols, DType dtype, hipStream_t s,
-                       CsrHost& out);  // csr.hip
+void coo_to_csr_device(const double* vals, const int4* pairs, int npairs, const int* test_dofs, int nbt, int tstride,
+                       const int* trial_dofs, int nbs, int sstride, int64_t nrows, int64_t ncols, DType dtype,
+                       hipStream_t s, CsrHost& out);  // csr.hip
 
 void assemble_singular_csr(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test, bool correction,
                            CsrHost& out) {
+  if (test.grid->general() || trial.grid->general()) {
+    check_assembler(a, trial, test);
+    assemble_cells_singular_csr(a, trial, test, correction, assembler_stream(a, current_device()), out);
+    return;
+  }
   out.indptr.assign((size_t)test.size + 1, 0);
   out.indices.clear();
   out.data.clear();
@@ -1173,8 +1178,8 @@ void assemble_singular_csr(const Assembler& a, const FunctionSpace& trial, const
   if (P.npairs == 0) return;
   double* d = adjacent_coo_values(a, trial, test, P, correction);
   try {
-    coo_to_csr_device(d, P.S.pairs, P.npairs, P.S.test_dofs, test.nb, P.S.trial_dofs, trial.nb, test.size, trial.size,
-                      a.dtype, P.stream, out);
+    coo_to_csr_device(d, P.S.pairs, P.npairs, P.S.test_dofs, test.nb, test.nb, P.S.trial_dofs, trial.nb, trial.nb,
+                      test.size, trial.size, a.dtype, P.stream, out);
   } catch (...) {
     (void)hipFree(d);
     throw;
@@ -1258,6 +1263,7 @@ void assemble_potential_device(const Assembler& a, const FunctionSpace& space_c,
 
 void assemble_potential_host(const Assembler& a, const FunctionSpace& space, const double* points, int64_t npoints,
                              void* host_out) {
+  require_triangles(space, "potential assembly");
   if (npoints == 0) return;
   if (!points || !host_out) throw Error("points or output pointer is null");
   if (a.kind == kHelmholtz) {  // the device sine/cosine reduction is exact while k r <= 1e8

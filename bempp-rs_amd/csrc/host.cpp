@@ -174,7 +174,9 @@ FunctionSpace::FunctionSpace(const Grid* g, int deg, bool cont) : grid(g), degre
 
 void require_triangles(const FunctionSpace& s, const char* what) {
   if (s.grid->general())
-    throw Error(std::string(what) + " is not supported on grids with quadrilateral cells (dense assembly is)");
+    throw Error(std::string(what) +
+                " is not supported on grids with quadrilateral cells (dense assembly and the sparse singular part and "
+                "correction are)");
 }
 
 FunctionSpace::~FunctionSpace() {

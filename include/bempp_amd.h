@@ -133,8 +133,10 @@ void grid_cells_copy(const GridT* grid, int64_t* cells);
  * f64; cell_types[c] BEMPP_CELL_TRIANGLE (3 vertices) or BEMPP_CELL_QUADRILATERAL (4
  * vertices, tensor order: reference (0,0), (1,0), (0,1), (1,1)); cell_vertices the cells'
  * vertices back to back. Dense assembly (assemble_into_memory, assemble_into_device_memory)
- * supports these grids in FP64 (the regular rule of quadrilaterals is m x m Gauss: its
- * point count must be a square); the other entry points refuse them. */
+ * and the sparse singular part and correction (assemble_singular,
+ * assemble_singular_correction) support these grids in FP64 (the regular rule of
+ * quadrilaterals is m x m Gauss: its point count must be a square); the other entry points
+ * refuse them. */
 GridT* grid_new_cells(const double* points, size_t npoints, const int64_t* cell_vertices, const uint8_t* cell_types,
                       size_t ncells);
 /* Number of entries of the cells' vertex lists, the lists back to back, and the cell types. */

@@ -1257,9 +1257,11 @@ bool cells_touch(const CGrid& g, int64_t a, int64_t b) {
 }
 
 // dense test x trial matrix (column-major) = regular part + singular part
+// part: 0 both parts, 1 the regular (non-adjacent) pairs only, 2 the adjacent pairs only,
+// 3 the adjacent pairs under the regular rule (the singular correction)
 template <typename T>
 void assemble_cells_t(int kind, int op, double k, const CGrid& g, const CSpace& test, const CSpace& trial, int nq_tri,
-                      int nq_quad, int nsing, int nthreads, T* out) {
+                      int nq_quad, int nsing, int nthreads, int part, T* out) {
   const bool helm = kind == 1;
   std::vector<double> rp[2], rw[2];
   cell_regular_rule(3, nq_tri, rp[0], rw[0]);
@@ -1276,21 +1278,26 @@ void assemble_cells_t(int kind, int op, double k, const CGrid& g, const CSpace& 
   // rows of one test cell are written by one thread: split the work by test DOF owner
   // through a per-thread output, summed at the end
   const int nt = std::max(1, nthreads);
-  std::vector<std::vector<T>> part(nt, std::vector<T>((size_t)ld * trial.size, T(0.0)));
+  std::vector<std::vector<T>> partial(nt, std::vector<T>((size_t)ld * trial.size, T(0.0)));
 #pragma omp parallel for schedule(dynamic, 4) num_threads(nt)
   for (int64_t tc = 0; tc < g.nc; ++tc) {
-    T* o = part[omp_get_thread_num()].data();
+    T* o = partial[omp_get_thread_num()].data();
     const int rt = g.nvc[tc] == 3 ? 0 : 1;
     for (int64_t sc = 0; sc < g.nc; ++sc) {
       T loc[4][4];
       for (auto& row : loc)
         for (auto& e : row) e = T(0.0);
-      if (!cells_touch(g, tc, sc)) {
+      const bool touch = cells_touch(g, tc, sc);
+      if ((touch && part == 1) || (!touch && part >= 2)) continue;
+      if (!touch || part == 3) {
         const int rs = g.nvc[sc] == 3 ? 0 : 1;
         for (size_t p = 0; p < rw[rt].size(); ++p) {
           const CPoint& P = tpts[tc][p];
           for (size_t q = 0; q < rw[rs].size(); ++q) {
             const CPoint& Q = spts[sc][q];
+            // the correction leaves out coincident points, as a point-to-point fast
+            // multipole evaluation does
+            if (part == 3 && P.x[0] == Q.x[0] && P.x[1] == Q.x[1] && P.x[2] == Q.x[2]) continue;
             T kv[4];
             kernel_at<T>(helm, k, P.x, Q.x, kv);
             const double wt = rw[rt][p] * P.jdet * rw[rs][q] * Q.jdet;
@@ -1316,7 +1323,7 @@ void assemble_cells_t(int kind, int op, double k, const CGrid& g, const CSpace& 
     }
   }
   for (int t = 0; t < nt; ++t)
-    for (size_t i = 0; i < part[t].size(); ++i) out[i] += part[t][i];
+    for (size_t i = 0; i < partial[t].size(); ++i) out[i] += partial[t][i];
 }
 
 }  // namespace
@@ -1327,14 +1334,18 @@ extern "C" {
 // (column-major test x trial; complex interleaved for Helmholtz). cell_vertices: the
 // cells' vertex lists back to back (3 per triangle, 4 per quadrilateral in tensor order);
 // cell_types: 2 triangle, 3 quadrilateral. nq_tri / nq_quad: regular points per cell type.
+// part: 0 the whole operator, 1 its regular (non-adjacent) pairs, 2 its adjacent pairs, 3 the
+// adjacent pairs under the regular rule.
 int oracle_assemble_cells(int kind, int op, double k, const double* pts, int64_t nv, const int64_t* cell_vertices,
                           const uint8_t* cell_types, int64_t nc, int test_degree, int test_cont, int trial_degree,
-                          int trial_cont, int nq_tri, int nq_quad, int nsing, int nthreads, void* out) {
+                          int trial_cont, int nq_tri, int nq_quad, int nsing, int nthreads, int part, void* out) {
   try {
     CGrid g(pts, nv, cell_vertices, cell_types, nc);
     CSpace ts(&g, test_degree, test_cont != 0), ss(&g, trial_degree, trial_cont != 0);
-    if (kind == 0) assemble_cells_t<double>(kind, op, k, g, ts, ss, nq_tri, nq_quad, nsing, nthreads, (double*)out);
-    else assemble_cells_t<cplx>(kind, op, k, g, ts, ss, nq_tri, nq_quad, nsing, nthreads, (cplx*)out);
+    if (kind == 0)
+      assemble_cells_t<double>(kind, op, k, g, ts, ss, nq_tri, nq_quad, nsing, nthreads, part, (double*)out);
+    else
+      assemble_cells_t<cplx>(kind, op, k, g, ts, ss, nq_tri, nq_quad, nsing, nthreads, part, (cplx*)out);
     return 0;
   } catch (const std::exception& e) {
     g_err = e.what();

@@ -63,3 +63,73 @@ def regular_parts(kind, op, k, points, cells, dofs, ndofs, npts=6):
     P = basis_to_quadrature(dofs, ndofs, xi, wj)
     K = kernel_matrix(kind, op, k, x.reshape(-1, 3), np.repeat(n, Q, axis=0))
     return P.T @ K @ P, P.T @ np.where(adjacency_mask(cells, Q), K, 0.0) @ P
+
+
+def _cell_points(points, cell, xi):
+    """Mapped points, Jacobians |x_s x x_t|, unit normals, basis values and J-scaled surface
+    curls (t_t d_s phi - t_s d_t phi) of a triangle or bilinear quadrilateral at reference points xi."""
+    V = points[cell]
+    s, t = xi[:, 0:1], xi[:, 1:2]
+    e1, e2 = V[1] - V[0], V[2] - V[0]
+    tw = V[3] - V[1] - V[2] + V[0] if len(cell) == 4 else np.zeros(3)
+    X = V[0] + s * e1 + t * e2 + s * t * tw
+    ts, tt = e1 + t * tw, e2 + s * tw
+    cr = np.cross(ts, tt)
+    J = np.linalg.norm(cr, axis=1)
+    s, t = s[:, 0], t[:, 0]
+    if len(cell) == 4:
+        phi = np.stack([(1 - s) * (1 - t), s * (1 - t), (1 - s) * t, s * t], axis=1)
+        ds = np.stack([-(1 - t), 1 - t, -t, t], axis=1)
+        dt = np.stack([-(1 - s), -s, 1 - s, s], axis=1)
+    else:
+        phi = np.stack([1 - s - t, s, t], axis=1)
+        ds = np.tile([-1.0, 1.0, 0.0], (len(s), 1))
+        dt = np.tile([-1.0, 0.0, 1.0], (len(s), 1))
+    curl = tt[:, None, :] * ds[:, :, None] - ts[:, None, :] * dt[:, :, None]
+    return X, J, cr / J[:, None], phi, curl
+
+
+def cells_regular_parts(kind, op, k, points, cells, degree, continuous, nq_tri=6, m=4):
+    """On a grid of triangles and quadrilaterals (m x m Gauss-Legendre points per
+    quadrilateral, the oracle's nq_tri-point rule per triangle): (P^T K P over all cell pairs,
+    the same over adjacent pairs only), K zero at zero distance; degree 0 or 1 (P1 / Q1)."""
+    g, gw = np.polynomial.legendre.leggauss(m)
+    g, gw = 0.5 * (g + 1.0), 0.5 * gw
+    qxi = np.array([[a, b] for a in g for b in g])
+    qw = np.array([wa * wb for wa in gw for wb in gw])
+    txi, tw = O.regular_rule(nq_tri)
+    ndofs, dofs = O.cells_space(points, cells, degree, continuous)
+    X, WJ, N, W, owner, PHI, CURL = [], [], [], [], [], [], []
+    for c, cell in enumerate(cells):
+        xi, w = (qxi, qw) if len(cell) == 4 else (txi, tw)
+        x, J, n, phi, curl = _cell_points(points, np.asarray(cell), xi)
+        if degree == 0:
+            phi, curl = np.ones((len(w), 1)), np.zeros((len(w), 1, 3))
+        for lst, v in ((X, x), (WJ, w * J), (N, n), (W, w), (PHI, phi), (CURL, curl)):
+            lst.append(v)
+        owner += [c] * len(w)
+    X, N = np.concatenate(X), np.concatenate(N)
+    owner = np.array(owner)
+    K = kernel_matrix(kind, "single_layer" if op == "hypersingular" else op, k, X, N)
+    npt = len(X)
+    P = np.zeros((npt, ndofs))
+    Pc = np.zeros((3, npt, ndofs))
+    row = 0
+    for c in range(len(cells)):
+        nq = len(WJ[c])
+        for b, dof in enumerate(dofs[c]):
+            P[row:row + nq, dof] += PHI[c][:, b] * WJ[c]
+            Pc[:, row:row + nq, dof] += (CURL[c][:, b, :] * W[c][:, None]).T
+        row += nq
+    adjacent = np.array([[bool(set(cells[a]) & set(cells[b])) for b in range(len(cells))] for a in range(len(cells))])
+    mask = adjacent[owner][:, owner]
+
+    def form(Km):
+        if op != "hypersingular":
+            return P.T @ Km @ P
+        out = sum(Pc[d].T @ Km @ Pc[d] for d in range(3))
+        if kind == "helmholtz":
+            out = out - k * k * sum((P * N[:, d:d + 1]).T @ Km @ (P * N[:, d:d + 1]) for d in range(3))
+        return out
+
+    return form(K), form(np.where(mask, K, 0.0))

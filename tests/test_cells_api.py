@@ -10,6 +10,7 @@ import pytest
 import oracle_lib as O
 from bempp_amd import _lib as L
 from bempp_amd.assembly.boundary import BoundaryAssemblerOptions, OperatorType, create_laplace_assembler
+from bempp_amd.assembly.potential import create_laplace_potential_assembler
 from bempp_amd.element import Continuity, Family, ReferenceCellType, create_family
 from bempp_amd.function_space import function_space
 from bempp_amd.grid import Grid
@@ -76,6 +77,8 @@ def test_ndgrid_ingest_of_quadrilateral_grid():
 
 
 def test_triangle_only_entry_points_refuse_quadrilateral_grids():
+    """Matrix-free apply, the shape gradient and potentials are specialised to triangles:
+    they refuse such grids before any device work."""
     pts, quads = cube(1)
     g = Grid(pts, np.array(quads))
     s = function_space(g, create_family(Family.Lagrange, 0, Continuity.Discontinuous))
@@ -83,6 +86,9 @@ def test_triangle_only_entry_points_refuse_quadrilateral_grids():
     o.set_regular_quadrature_degree(ReferenceCellType.Quadrilateral, 16)
     a = create_laplace_assembler(OperatorType.SingleLayer, o)
     with pytest.raises(ValueError, match="quadrilateral"):
-        a.assemble_singular(s, s)
+        a.apply(s, s, np.ones(s.global_size))
     with pytest.raises(ValueError, match="quadrilateral"):
-        a.assemble_singular_correction(s, s)
+        a.shape_gradient(s, s, np.ones(s.global_size), np.ones(s.global_size))
+    pa = create_laplace_potential_assembler(OperatorType.SingleLayer, o)
+    with pytest.raises(ValueError, match="quadrilateral"):
+        pa.assemble(s, np.array([[3.0, 0.0, 0.0]]))

@@ -163,3 +163,51 @@ def test_larger_cubed_sphere():
     err = np.linalg.norm(A - R) / np.linalg.norm(R)
     print(f"[cells] cubed sphere 12: rel err {err:.2e}", flush=True)
     assert err <= 1e-12
+
+
+def _adjacent_pattern(pts, cells, test, trial):
+    """Row-major keys row * ncols + col of the DOF pairs of adjacent cells (sorted, unique)."""
+    nt, tdofs = O.cells_space(pts, cells, *SPACES[test])
+    ns, sdofs = O.cells_space(pts, cells, *SPACES[trial])
+    keys = set()
+    for a in range(len(cells)):
+        for b in range(len(cells)):
+            if set(cells[a]) & set(cells[b]):
+                keys.update(r * ns + c for r in tdofs[a] for c in sdofs[b])
+    return np.array(sorted(keys), dtype=np.int64), nt, ns
+
+
+@pytest.mark.parametrize("mesh", ["cubed_sphere", "mixed_cube", "mixed_screen"])
+@pytest.mark.parametrize("kind,k", [("laplace", 0.0), ("helmholtz", 2.0)])
+@pytest.mark.parametrize("op", list(OPS))
+def test_quadrilateral_singular_csr(mesh, kind, k, op):
+    """assemble_singular and assemble_singular_correction (src/boundary_assemblers.rs:122-169)
+    on grids with quadrilaterals: CSR pattern = the DOF pairs of adjacent cells (a triangle's
+    missing fourth function leaves no entry), columns strictly ascending per row, the same
+    bytes on a second call, values against the oracle's adjacent-pair parts (pair rules /
+    regular rule without coincident points)."""
+    pts, cells = _mesh(mesh)
+    g = Grid(pts, cells)
+    o = _options()
+    a = _assembler(kind, op, k, o)
+    for test, trial in [("dp0", "dp0"), ("p1", "p1"), ("dp1", "p1"), ("dp0", "dp1")]:
+        st, ss = _space(g, test), _space(g, trial)
+        keys, nt, ns = _adjacent_pattern(pts, cells, test, trial)
+        for part, fn in [("singular", a.assemble_singular), ("correction", a.assemble_singular_correction)]:
+            print(f"[cells-csr] {mesh} {kind} {op} {test}x{trial} {part}", flush=True)
+            csr = fn(ss, st)
+            again = fn(ss, st)
+            assert np.array_equal(csr.indptr, again.indptr) and np.array_equal(csr.indices, again.indices)
+            assert np.array_equal(csr.data.view(np.uint8), again.data.view(np.uint8))
+            R = O.assemble_cells(kind, op, pts, cells, test=SPACES[test], trial=SPACES[trial], k=k, nq_tri=6,
+                                 nq_quad=16, nsing=4, part=part)
+            assert csr.shape == (nt, ns) and csr.indptr[0] == 0 and csr.indptr[-1] == csr.nnz
+            if np.abs(R).max() <= 1e-10:  # every entry at most 1e-10: the empty matrix
+                assert csr.nnz == 0
+                continue
+            row_of = np.repeat(np.arange(nt), np.diff(csr.indptr))
+            assert np.all(np.diff(csr.indices)[np.diff(row_of) == 0] > 0)
+            assert np.array_equal(row_of * ns + csr.indices, keys)
+            err = np.linalg.norm(csr.toarray() - R) / np.linalg.norm(R)
+            print(f"[cells-csr]   nnz {csr.nnz}, rel err {err:.2e}", flush=True)
+            assert err <= 1e-12

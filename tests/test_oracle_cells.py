@@ -163,3 +163,50 @@ def test_cells_space_numbering():
     assert n == 20 and dofs[1] == [4, 5, 6, 7] and dofs[5] == [17, 18, 19]
     with pytest.raises(ValueError, match="square number"):
         O.assemble_cells("laplace", "single_layer", pts, mixed, nq_quad=37)
+
+
+def _cube_mixed():
+    pts, c = cube(2)
+    return pts, split(c, lambda i: i % 3 == 0)
+
+
+@pytest.mark.parametrize("kind,k", [("laplace", 0.0), ("helmholtz", 2.5)])
+@pytest.mark.parametrize("op", OPS)
+def test_parts_split_the_operator(kind, k, op):
+    """The oracle's parts: whole operator = regular (non-adjacent pairs) + singular (adjacent
+    pairs, pair rules); the singular part lives on the DOF pairs of adjacent cells."""
+    pts, cells = _cube_mixed()
+    for sp in [(0, False), (1, True)]:
+        A = O.assemble_cells(kind, op, pts, cells, test=sp, trial=sp, k=k)
+        R = O.assemble_cells(kind, op, pts, cells, test=sp, trial=sp, k=k, part="regular")
+        S = O.assemble_cells(kind, op, pts, cells, test=sp, trial=sp, k=k, part="singular")
+        assert np.abs(A - R - S).max() <= 1e-15 * max(np.abs(A).max(), 1e-300)
+        _, dofs = O.cells_space(pts, cells, *sp)
+        allowed = np.zeros(A.shape, dtype=bool)
+        for a in range(len(cells)):
+            for b in range(len(cells)):
+                if set(cells[a]) & set(cells[b]):
+                    allowed[np.ix_(dofs[a], dofs[b])] = True
+        assert not np.any(S[~allowed])
+
+
+@pytest.mark.parametrize("kind,k", [("laplace", 0.0), ("helmholtz", 2.5)])
+@pytest.mark.parametrize("op", OPS)
+@pytest.mark.parametrize("mesh", ["cubed_sphere", "mixed_cube"])
+def test_regular_and_correction_parts_against_numpy(mesh, kind, k, op):
+    """The regular rule written out in numpy (bilinear map, Q1 / P1 bases, surface curls,
+    kernel zero at zero distance): the correction part (adjacent pairs under the regular rule)
+    equals the numpy sums over adjacent cell pairs, regular + correction the sums over all."""
+    import fmm_helpers as F
+
+    pts, cells = cubed_sphere(2) if mesh == "cubed_sphere" else _cube_mixed()
+    for deg, cont in [(0, False), (1, True)]:
+        full, adj = F.cells_regular_parts(kind, op, k, pts, cells, deg, cont, nq_tri=6, m=4)
+        kw = dict(test=(deg, cont), trial=(deg, cont), k=k, nq_tri=6, nq_quad=16)
+        C = O.assemble_cells(kind, op, pts, cells, part="correction", **kw)
+        R = O.assemble_cells(kind, op, pts, cells, part="regular", **kw)
+        if np.linalg.norm(full) == 0.0:  # Laplace hypersingular on P0
+            assert np.abs(C).max() == 0.0 and np.abs(R).max() == 0.0
+            continue
+        assert np.linalg.norm(C - adj) <= 1e-14 * np.linalg.norm(adj)
+        assert np.linalg.norm(R + C - full) <= 1e-14 * np.linalg.norm(full)
