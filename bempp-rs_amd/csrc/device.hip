@@ -790,7 +790,9 @@ Prepared prepare(const Assembler& a, const FunctionSpace& trial_c, const Functio
   // |k (r - R)| <= |k| (rad_test + rad_trial) for every quadrature-point pair of a
   // regular cell pair; the small-argument sine/cosine needs this bound <= 0.5.
   const double phase_bound = std::fabs(a.k) * (tc->max_radius + sc->max_radius);
-  P.centred = a.kind == kHelmholtz && phase_bound <= 0.5;
+  // (|k| below 1e-100, k = 0 included, takes the full reduction: the centred tile sums work
+  // in lengths scaled by |k|)
+  P.centred = a.kind == kHelmholtz && std::fabs(a.k) >= 1e-100 && phase_bound <= 0.5;
   P.tier = !P.centred ? 0 : (phase_bound <= 0.25 ? 2 : 1);
   // points of two cells sharing a vertex are at most 2 (rad_i + rad_j) apart
   P.small_adjacent = a.kind == kHelmholtz && P.same_grid && std::fabs(a.k) * 4.0 * tc->max_radius <= 0.5;

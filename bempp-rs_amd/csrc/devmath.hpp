@@ -85,29 +85,33 @@ __device__ __forceinline__ void sincos_cw(double x, double* s_out, double* c_out
 // 4/5 (tier 1) and 3/4 (tier 2), computed by Lawson's iteration at 50 digits; the
 // truncation errors are below 1e-17 and the float64 Horner evaluation errors below
 // 6e-17 (sin) and 1.2e-16 (cos) on the whole interval.
-template <int TIER = 1>
+// SQRT2 = true: the argument arrives multiplied by sqrt 2, x = sqrt(2) a, and the results are
+// cos a and sqrt(2) sin a: the same polynomials in u = x^2 = 2 a^2 with the coefficient of u^n
+// divided by 2^(n+1) (exact), i.e. sin a / a = 1 + (u/2) P(u/2), cos a = 1 + (u/2) C(u/2).
+template <int TIER = 1, bool SQRT2 = false>
 __device__ __forceinline__ void sincos_small(double x, double* s_out, double* c_out) {
   static_assert(TIER == 1 || TIER == 2, "phase tiers: 1 (|x| <= 0.5), 2 (|x| <= 0.25)");
+  constexpr double f1 = SQRT2 ? 0.5 : 1.0, f2 = f1 * f1, f3 = f2 * f1, f4 = f2 * f2, f5 = f4 * f1, f6 = f3 * f3;
   const double u = x * x;
   double p, c;
   if constexpr (TIER == 1) {
-    p = fma(u, -2.4927410039626402e-08, 2.7556952897882737e-06);
-    p = fma(p, u, -0.00019841269341158048);
-    p = fma(p, u, 0.008333333333023885);
-    p = fma(p, u, -0.16666666666666);
-    c = fma(u, 2.077788258235537e-09, -2.7556985499996286e-07);
-    c = fma(c, u, 2.480158674730518e-05);
-    c = fma(c, u, -0.00138888888884234);
-    c = fma(c, u, 0.04166666666666487);
-    c = fma(c, u, -0.5);
+    p = fma(u, -2.4927410039626402e-08 * f5, 2.7556952897882737e-06 * f4);
+    p = fma(p, u, -0.00019841269341158048 * f3);
+    p = fma(p, u, 0.008333333333023885 * f2);
+    p = fma(p, u, -0.16666666666666 * f1);
+    c = fma(u, 2.077788258235537e-09 * f6, -2.7556985499996286e-07 * f5);
+    c = fma(c, u, 2.480158674730518e-05 * f4);
+    c = fma(c, u, -0.00138888888884234 * f3);
+    c = fma(c, u, 0.04166666666666487 * f2);
+    c = fma(c, u, -0.5 * f1);
   } else {
-    p = fma(u, 2.751683000751954e-06, -0.0001984124679978689);
-    p = fma(p, u, 0.00833333332798428);
-    p = fma(p, u, -0.1666666666666255);
-    c = fma(u, -2.751887867547197e-07, 2.4801560959230762e-05);
-    c = fma(c, u, -0.0013888888880693657);
-    c = fma(c, u, 0.04166666666665554);
-    c = fma(c, u, -0.49999999999999994);
+    p = fma(u, 2.751683000751954e-06 * f4, -0.0001984124679978689 * f3);
+    p = fma(p, u, 0.00833333332798428 * f2);
+    p = fma(p, u, -0.1666666666666255 * f1);
+    c = fma(u, -2.751887867547197e-07 * f5, 2.4801560959230762e-05 * f4);
+    c = fma(c, u, -0.0013888888880693657 * f3);
+    c = fma(c, u, 0.04166666666665554 * f2);
+    c = fma(c, u, -0.49999999999999994 * f1);
   }
   *s_out = fma(x * u, p, x);
   *c_out = fma(u, c, 1.0);

@@ -164,7 +164,8 @@ __device__ __forceinline__ cplx kernel_scalar(double dx, double dy, double dz, d
 
 // acc += w G for the single-layer kernel (without 1/(4 pi)); the weight is folded into
 // 1/r so that the complex update costs one product and two fused multiply-adds.
-template <bool HELM, int CENTRED>
+// WEIGHTED = false: w is ignored and the unweighted kernel value is added
+template <bool HELM, int CENTRED, bool WEIGHTED = true>
 __device__ __forceinline__ void green_accumulate_r2(double r2, double k, double kR, double w, cplx& acc) {
   const double rinv = rsqrt_fast(r2);
   if constexpr (HELM) {
@@ -174,7 +175,7 @@ __device__ __forceinline__ void green_accumulate_r2(double r2, double k, double 
       sincos_small<CENTRED>(fma(k, r, -kR), &s, &c);
     else
       sincos_cw(k * r, &s, &c);
-    const double rw = rinv * w;
+    const double rw = WEIGHTED ? rinv * w : rinv;
     acc.re = fma(c, rw, acc.re);
     acc.im = fma(s, rw, acc.im);
   } else {
@@ -627,6 +628,13 @@ __device__ __forceinline__ cplx p0_pair(const double (&xp)[Q][3], const double (
 // two instructions per point pair instead of six (three differences and a dot product);
 // W_p2 = -W_p0 - W_p1 and the d'_2 products follow the same way, the offsets of a cell
 // summing to zero. tau2g[p] = tau_p^2 |d'_k'(p)|^2; sv: the trial cell's 12-value record.
+// The centred Helmholtz single-layer (and P0 hypersingular) orbit sums of rules with at most
+// two weight classes run in scaled lengths: the caller multiplies the test cell's vertex
+// offsets and tau_p^2 |d'_k'|^2 by k^2 / 2 (p0_pair_orbit scales the rest).
+__host__ __device__ constexpr bool orbit_scaled(int Q, int OP, bool HELM, int CENTRED) {
+  return HELM && (OP == 0 || OP == 3) && CENTRED > 0 && weight_classes(Q) <= 2;
+}
+
 template <int Q, int OP, bool HELM, int CENTRED, typename SV>
 __device__ __forceinline__ cplx p0_pair_orbit(const double (&td)[2][3], const double (&tau2g)[Q],
                                               const double (&twc)[3], const SV& sv, const double (&D)[3], double R2,
@@ -658,8 +666,12 @@ __device__ __forceinline__ cplx p0_pair_orbit(const double (&td)[2][3], const do
       dnq[q] = t == 0.0 ? Dn : fma(t, dkn[orbit_vertex(Q, q)], Dn);
     }
   }
-  const double E0 = -2.0 * (D[0] * sv[0] + D[1] * sv[1] + D[2] * sv[2]);
-  const double E1 = -2.0 * (D[0] * sv[4] + D[1] * sv[5] + D[2] * sv[6]);
+  // orbit_scaled: td and tau2g arrive multiplied by sc = k^2 / 2; E, R^2 and the |d_k|^2
+  // terms are scaled here, so that every r^2 below is sc r^2
+  constexpr bool SCALED = orbit_scaled(Q, OP, HELM, CENTRED);
+  const double sc = SCALED ? 0.5 * k * k : 1.0;
+  const double E0 = (SCALED ? -k * k : -2.0) * (D[0] * sv[0] + D[1] * sv[1] + D[2] * sv[2]);
+  const double E1 = (SCALED ? -k * k : -2.0) * (D[0] * sv[4] + D[1] * sv[5] + D[2] * sv[6]);
   double N[3][2], F[3];
 #pragma unroll
   for (int m = 0; m < 2; ++m) {
@@ -671,11 +683,68 @@ __device__ __forceinline__ cplx p0_pair_orbit(const double (&td)[2][3], const do
   N[2][1] = -(N[0][1] + N[1][1]);
   F[2] = -(F[0] + F[1]);
   const double gk[3] = {sv[3], sv[7], sv[11]};
+  const double R2s = SCALED ? sc * R2 : R2;
   double RC[Q];
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const double t = orbit_t(Q, q);
-    RC[q] = t == 0.0 ? R2 : fma(t * t, gk[orbit_vertex(Q, q)], R2);
+    RC[q] = t == 0.0 ? R2s : fma(SCALED ? t * t * sc : t * t, gk[orbit_vertex(Q, q)], R2s);
+  }
+  if constexpr ((OP == 0 || OP == 3) && HELM && kWeightClasses<Q> <= 2) {
+    // unweighted sums per (test weight class, trial weight class), the class weights applied
+    // once per cell pair: no weight multiply per point pair or per test point
+    constexpr int NC = kWeightClasses<Q>;
+    const double kRs = SCALED ? 1.4142135623730951 * fabs(kR) : 0.0;  // sqrt(2) |k| R
+    cplx cacc[NC][NC];
+#pragma unroll
+    for (int a = 0; a < NC; ++a)
+#pragma unroll
+      for (int b = 0; b < NC; ++b) cacc[a][b] = {0.0, 0.0};
+#pragma unroll
+    for (int p = 0; p < Q; ++p) {
+      const int kp = orbit_vertex(Q, p);
+      const double tau = orbit_t(Q, p);
+      double W[3];
+      W[0] = tau == 0.0 ? E0 : fma(-2.0 * tau, N[kp][0], E0);
+      W[1] = tau == 0.0 ? E1 : fma(-2.0 * tau, N[kp][1], E1);
+      W[2] = -(W[0] + W[1]);
+      const double Ap = fma(tau, F[kp], tau2g[p]);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const double t = orbit_t(Q, q);
+        const double base = tau == 0.0 ? RC[q] : Ap + RC[q];
+        const double r2 = t == 0.0 ? base : fma(t, W[orbit_vertex(Q, q)], base);
+        cplx& ca = cacc[weight_class(Q, p)][weight_class(Q, q)];
+        if constexpr (SCALED) {
+          // h = sc r^2, y ~ h^(-1/2): one Newton step y (3 - h y^2) without its factor 1/2 gives
+          // rz = 2 sqrt(2) / (|k| r), and h rz = sqrt(2) |k| r (the phase in one FMA)
+          const double y = __builtin_amdgcn_rsq(r2);
+          const double rz = y * fma(-(r2 * y), y, 3.0);
+          double sn, cs;
+          sincos_small<CENTRED, true>(fma(r2, rz, -kRs), &sn, &cs);
+          ca.re = fma(cs, rz, ca.re);
+          ca.im = fma(sn, rz, ca.im);
+        } else {
+          green_accumulate_r2<HELM, CENTRED, false>(r2, k, kR, 1.0, ca);
+        }
+      }
+    }
+    cplx acc = {0.0, 0.0};
+#pragma unroll
+    for (int a = 0; a < NC; ++a)
+#pragma unroll
+      for (int b = 0; b < NC; ++b) {
+        const double w = twc[a] * sv[8 + b];
+        acc.re = fma(cacc[a][b].re, w, acc.re);
+        acc.im = fma(cacc[a][b].im, w, acc.im);
+      }
+    if constexpr (SCALED) {
+      // the sums hold 2 sqrt(2) cos(a) / (|k| r) and 4 sin(a) / (|k| r), a = |k| (r - R);
+      // sin(k (r - R)) = sign(k) sin a
+      acc.re *= fabs(k) * (0.25 * 1.4142135623730951);
+      acc.im *= 0.25 * k;
+    }
+    return acc;
   }
   cplx acc = {0.0, 0.0};
 #pragma unroll
@@ -941,6 +1010,15 @@ __device__ __forceinline__ void p0_tile(Args& A, const int2 tile, int symmetric,
       tau2g[p] = tau * tau * tg[orbit_vertex(Q, p)];
     }
     for (int c = 0; c < 3; ++c) twc[c] = tv[8 + c];
+    if constexpr (orbit_scaled(Q, OP, HELM, CENTRED)) {  // lengths scaled by |k| / sqrt 2
+      const double sc = 0.5 * A.k * A.k;
+#pragma unroll
+      for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) td[m][d] *= sc;
+#pragma unroll
+      for (int p = 0; p < Q; ++p) tau2g[p] *= sc;
+    }
   }
   const double* tgeo = A.test_geo + (size_t)il * kGeoStride;
   const double nx[3] = {tgeo[9], tgeo[10], tgeo[11]};
@@ -1024,9 +1102,11 @@ __device__ __forceinline__ void p0_tile(Args& A, const int2 tile, int symmetric,
   }
 }
 
+// the scaled orbit sums fit three waves per SIMD (at most 168 VGPRs)
 template <int Q, int OP, bool HELM, int CENTRED>
-__global__ void __launch_bounds__(256) regular_p0_tiled(const RegularArgs A, const int2* __restrict__ tiles,
-                                                        int symmetric) {
+__global__ void __launch_bounds__(256)
+    __attribute__((amdgpu_waves_per_eu(orbit_scaled(Q, OP, HELM, CENTRED) ? 3 : 1)))
+    regular_p0_tiled(const RegularArgs A, const int2* __restrict__ tiles, int symmetric) {
   __shared__ double2 lds[4][kChunk][kTile];
   p0_tile<Q, OP, HELM, CENTRED>(A, tiles[blockIdx.x], symmetric, lds);
 }

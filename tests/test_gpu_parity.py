@@ -472,3 +472,25 @@ def test_oversized_host_assembly_fails_cleanly():
     A = a.assemble(sp, sp)
     R = O.assemble("laplace", "single_layer", small.points, small.cells)
     assert _rel(A, R) <= TOL
+
+
+@pytest.mark.parametrize("k", [2.5, 5.0, -2.5, -5.0, 1e-120, 0.0])
+@pytest.mark.parametrize("op", ["single_layer", "hypersingular"])
+def test_fp64_p0_scaled_orbit_sums(op, k):
+    """The FP64 P0 tiles' Helmholtz single-layer / P0 hypersingular sums in lengths scaled by
+    |k|/sqrt(2) (level-4 icosphere: k = +-2.5 phase tier 2, +-5 tier 1; the sine changes sign
+    with k) and the full-reduction path that |k| < 1e-100 takes: relative Frobenius error
+    <= 1e-12 against the oracle."""
+    g = icosphere(4)
+    s = _space(g, "dp0")
+    a = _assembler("helmholtz", op, k, _options())
+    print(f"[scaled] {op} k={k}", flush=True)
+    A = a.assemble(s, s)
+    R = O.assemble("helmholtz", op, g.points, g.cells, k=k)
+    scale = np.abs(R).max()
+    if scale == 0.0:  # P0 hypersingular at k = 0: -k^2 n.n vanishes
+        assert np.abs(A).max() == 0.0
+        return
+    err = _rel(A / scale, R / scale)  # entries of order k^2 = 1e-240 at k = 1e-120
+    print(f"[scaled]   rel err {err:.2e}", flush=True)
+    assert err <= 1e-12
