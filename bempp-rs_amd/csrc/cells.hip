@@ -24,6 +24,7 @@
 #include <map>
 #include <string>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "core.hpp"
@@ -62,6 +63,8 @@ struct CellsArgs {
   int npairs;
   int correction;      // adjacent pairs under the cells' regular rules instead of their pair rules
   double* coo;         // non-null: the adjacent pairs' values in COO layout instead of `out`
+  int max_trial_points;  // host: the most regular points of a trial cell (picks the staged kernel)
+  double radius_sum;     // host: largest test plus largest trial centroid-to-vertex distance
 };
 
 // Point, normal, Jacobian, basis values and J-scaled curls of a cell at (x, y); V: the
@@ -120,6 +123,10 @@ __device__ __forceinline__ void add_point_pair(const double* xp, const double* n
                                                double kR = 0.0) {
   const double dx = xp[0] - xq[0], dy = xp[1] - xq[1], dz = xp[2] - xq[2];
   const double r2 = dx * dx + dy * dy + dz * dz;
+  if constexpr (OP == 0 && NBT == 1 && NBS == 1) {  // P0 single layer: the kernel times w_p J_p w_q J_q
+    green_accumulate_r2<HELM, TIER>(r2, k, kR, wpq, M[0][0]);
+    return;
+  }
   double dn = 0.0;
   if constexpr (OP == 1) dn = dx * nq[0] + dy * nq[1] + dz * nq[2];
   if constexpr (OP == 2) dn = -(dx * np[0] + dy * np[1] + dz * np[2]);
@@ -156,10 +163,18 @@ __device__ __forceinline__ void cell_centroid(const V& cv, bool quad, double* c)
     c[d] = quad ? 0.25 * (cv[d] + cv[3 + d] + cv[6 + d] + cv[9 + d]) : (cv[d] + cv[3 + d] + cv[6 + d]) * (1.0 / 3.0);
 }
 
+// records of trial cells staged in LDS per workgroup (24 KB); cells with more points take
+// the unstaged kernel
+constexpr int kStageRecords = 128;
+
 // TIER (Helmholtz): 0 full argument reduction per point pair; 1, 2 the phase relative to the
 // centroid distance R, |k (r - R)| <= 0.5 / 0.25 over the whole grid pair.
-template <int OP, bool HELM, int NBT, int NBS, int TIER>
+// STAGED: the workgroup's four wavefronts walk the same trial cells (blockIdx.y's chunk), whose
+// records are contiguous: runs of whole cells (at most kStageRecords records) are copied into
+// LDS once and read back as broadcasts, instead of a scalar load (and wait) per trial point.
+template <int OP, bool HELM, int NBT, int NBS, int TIER, bool STAGED>
 __global__ void __launch_bounds__(256) cells_regular_kernel(const CellsArgs A) {
+  __shared__ double stage[STAGED ? kStageRecords * kRec : 1];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool i_ok = i < A.n_test;
   const int il = i_ok ? i : A.n_test - 1;
@@ -175,10 +190,10 @@ __global__ void __launch_bounds__(256) cells_regular_kernel(const CellsArgs A) {
   const auto svid = constant_view(A.svid);
   const auto sdofs = constant_view(A.sdofs);
   const auto sfirst = constant_view(A.sfirst);
-  const auto srec = constant_view(A.srec);
   const int j0 = blockIdx.y * A.trial_chunk;
   const int j1 = min(j0 + A.trial_chunk, A.n_trial);
-  for (int j = j0; j < j1; ++j) {
+  // trial cell j, its nq point records from Qj on
+  auto trial_cell = [&](int j, const auto* Qj, int nq) {
     bool adjacent = false;
     if (A.same_grid)
 #pragma unroll
@@ -187,7 +202,7 @@ __global__ void __launch_bounds__(256) cells_regular_kernel(const CellsArgs A) {
 #pragma unroll
         for (int a = 0; a < 4; ++a) adjacent = adjacent || (sv >= 0 && sv == tv[a]);
       }
-    if (adjacent || !i_ok) continue;
+    if (adjacent || !i_ok) return;
     cplx M[NBT][NBS];
 #pragma unroll
     for (int a = 0; a < NBT; ++a)
@@ -200,31 +215,78 @@ __global__ void __launch_bounds__(256) cells_regular_kernel(const CellsArgs A) {
       cell_centroid(scv + 12 * (size_t)j, svid[4 * j + 3] >= 0, cs);
       e = pair_phase(ct, cs, A.k, &kR);
     }
-    const int q0 = sfirst[j], q1 = sfirst[j + 1];
-    for (int p = p0; p < p1; ++p) {
-      const double* P = A.trec + (size_t)p * kRec;
-      const double xp[3] = {P[0], P[1], P[2]}, np[3] = {P[4], P[5], P[6]};
-      const double wJp = P[3], wp = P[7];
-      double php[NBT], cp[NBT][3];
+    // test points two at a time (PP = 2; one left over: PP = 1) when the local block is at most
+    // 4 entries: each trial point's record is read once for both, and their two point-pair
+    // chains interleave (the 4 x 4 blocks would need twice their registers)
+    auto test_points = [&](auto pp, int p) {
+      constexpr int PP = decltype(pp)::value;
+      double xp[PP][3], np[PP][3], wJp[PP], wp[PP], php[PP][NBT], cp[PP][NBT][3];
 #pragma unroll
-      for (int a = 0; a < NBT; ++a) {
-        php[a] = P[8 + a];
-        if constexpr (OP == 3)
+      for (int u = 0; u < PP; ++u) {
+        const double* P = A.trec + (size_t)(p + u) * kRec;
 #pragma unroll
-          for (int d = 0; d < 3; ++d) cp[a][d] = P[12 + 3 * a + d];
+        for (int d = 0; d < 3; ++d) {
+          xp[u][d] = P[d];
+          np[u][d] = P[4 + d];
+        }
+        wJp[u] = P[3];
+        wp[u] = P[7];
+#pragma unroll
+        for (int a = 0; a < NBT; ++a) {
+          php[u][a] = NBT == 1 ? 1.0 : P[8 + a];  // P0: the basis value 1 at compile time
+          if constexpr (OP == 3)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) cp[u][a][d] = P[12 + 3 * a + d];
+        }
       }
-      for (int q = q0; q < q1; ++q) {
-        const auto Q = srec + (size_t)q * kRec;
-        const double xq[3] = {Q[0], Q[1], Q[2]}, nq[3] = {Q[4], Q[5], Q[6]};
+      for (int q = 0; q < nq; ++q) {
+        const auto Q = Qj + (size_t)q * kRec;
+        const double xq[3] = {Q[0], Q[1], Q[2]}, nq3[3] = {Q[4], Q[5], Q[6]};
         double phq[NBS], cq[NBS][3];
 #pragma unroll
         for (int b = 0; b < NBS; ++b) {
-          phq[b] = Q[8 + b];
+          phq[b] = NBS == 1 ? 1.0 : Q[8 + b];
           if constexpr (OP == 3)
 #pragma unroll
             for (int d = 0; d < 3; ++d) cq[b][d] = Q[12 + 3 * b + d];
         }
-        add_point_pair<OP, HELM, NBT, NBS, TIER>(xp, np, php, cp, xq, nq, phq, cq, wJp * Q[3], wp * Q[7], A.k, M, kR);
+#pragma unroll
+        for (int u = 0; u < PP; ++u)
+          add_point_pair<OP, HELM, NBT, NBS, TIER>(xp[u], np[u], php[u], cp[u], xq, nq3, phq, cq, wJp[u] * Q[3],
+                                                   wp[u] * Q[7], A.k, M, kR);
+      }
+    };
+    if constexpr (NBT * NBS <= 4) {
+      int p = p0;
+      for (; p + 1 < p1; p += 2) test_points(std::integral_constant<int, 2>{}, p);
+      if (p < p1) test_points(std::integral_constant<int, 1>{}, p);
+    } else {
+      for (int p = p0; p < p1; ++p) {
+        const double* P = A.trec + (size_t)p * kRec;
+        const double xp[3] = {P[0], P[1], P[2]}, np[3] = {P[4], P[5], P[6]};
+        const double wJp = P[3], wp = P[7];
+        double php[NBT], cp[NBT][3];
+#pragma unroll
+        for (int a = 0; a < NBT; ++a) {
+          php[a] = P[8 + a];
+          if constexpr (OP == 3)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) cp[a][d] = P[12 + 3 * a + d];
+        }
+        for (int q = 0; q < nq; ++q) {
+          const auto Q = Qj + (size_t)q * kRec;
+          const double xq[3] = {Q[0], Q[1], Q[2]}, nq3[3] = {Q[4], Q[5], Q[6]};
+          double phq[NBS], cq[NBS][3];
+#pragma unroll
+          for (int b = 0; b < NBS; ++b) {
+            phq[b] = Q[8 + b];
+            if constexpr (OP == 3)
+#pragma unroll
+              for (int d = 0; d < 3; ++d) cq[b][d] = Q[12 + 3 * b + d];
+          }
+          add_point_pair<OP, HELM, NBT, NBS, TIER>(xp, np, php, cp, xq, nq3, phq, cq, wJp * Q[3], wp * Q[7], A.k, M,
+                                                   kR);
+        }
       }
     }
     if constexpr (TIER > 0)
@@ -242,6 +304,23 @@ __global__ void __launch_bounds__(256) cells_regular_kernel(const CellsArgs A) {
           store_value(A.out, (long long)td[a] + A.ld * col, {M[a][b].re * kInv4Pi, M[a][b].im * kInv4Pi}, A.out_type,
                       kAtomic);
     }
+  };
+  if constexpr (STAGED) {
+    // every lane (those past the last test cell included) takes part in the copies and barriers
+    for (int j = j0; j < j1;) {
+      const int base = sfirst[j];
+      int jend = j + 1;  // the host guarantees that one cell's records fit
+      while (jend < j1 && sfirst[jend + 1] - base <= kStageRecords) ++jend;
+      const int n = (sfirst[jend] - base) * kRec;
+      const double* src = A.srec + (size_t)base * kRec;
+      __syncthreads();
+      for (int t = threadIdx.x; t < n; t += blockDim.x) stage[t] = src[t];
+      __syncthreads();
+      for (; j < jend; ++j) trial_cell(j, stage + (sfirst[j] - base) * kRec, sfirst[j + 1] - sfirst[j]);
+    }
+  } else {
+    const auto srec = constant_view(A.srec);
+    for (int j = j0; j < j1; ++j) trial_cell(j, srec + (size_t)sfirst[j] * kRec, sfirst[j + 1] - sfirst[j]);
   }
 }
 
@@ -349,20 +428,17 @@ int cells_out_type(DType d) {
   }
 }
 
+// owns one device allocation; movable (vectors of them reallocate), not copyable
 struct DeviceBuf {
   void* p = nullptr;
+  DeviceBuf() = default;
+  DeviceBuf(const DeviceBuf&) = delete;
+  DeviceBuf& operator=(const DeviceBuf&) = delete;
+  DeviceBuf(DeviceBuf&& o) noexcept : p(o.p) { o.p = nullptr; }
   ~DeviceBuf() {
     if (p) (void)hipFree(p);
   }
 };
-
-template <typename T>
-T* upload_to(std::vector<DeviceBuf>& keep, const std::vector<T>& h, hipStream_t s) {
-  keep.emplace_back();
-  HIP_CHECK_Q(hipMalloc(&keep.back().p, std::max<size_t>(h.size(), 1) * sizeof(T)));
-  if (!h.empty()) HIP_CHECK_Q(hipMemcpyAsync(keep.back().p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
-  return (T*)keep.back().p;
-}
 
 // Point records of every cell of a space's grid for its regular rule (dev::kRec doubles each).
 void point_records(const FunctionSpace& sp, const Options& o, std::vector<double>& rec, std::vector<int>& first) {
@@ -414,20 +490,6 @@ void point_records(const FunctionSpace& sp, const Options& o, std::vector<double
   }
 }
 
-void cell_tables(const FunctionSpace& sp, std::vector<int>& vid, std::vector<int>& dofs, std::vector<double>& cv) {
-  const Grid& g = *sp.grid;
-  vid.assign(4 * g.nc(), -1);
-  dofs.assign(4 * g.nc(), -1);
-  cv.assign(12 * g.nc(), 0.0);
-  for (int64_t c = 0; c < g.nc(); ++c) {
-    for (int m = 0; m < g.cell_nv(c); ++m) {
-      vid[4 * c + m] = (int)g.vertex(c, m);
-      for (int d = 0; d < 3; ++d) cv[12 * c + 3 * m + d] = g.points[3 * g.vertex(c, m) + d];
-    }
-    for (int b = 0; b < sp.cell_nb(c); ++b) dofs[4 * c + b] = (int)sp.dofs[(size_t)sp.nb * c + b];
-  }
-}
-
 size_t singular_points(const Options& o, uint8_t t, uint8_t s) {
   auto it = o.singular_quadrature_degrees.find({t, s});
   if (it == o.singular_quadrature_degrees.end()) throw Error("no singular quadrature degree set for this cell pair type");
@@ -471,11 +533,21 @@ void adjacent_cell_pairs(const Grid& g, const Options& o, std::vector<int4>& pai
 }
 
 // tier < 0: the adjacent pairs only (no regular kernel)
+template <int OP, bool HELM, int NBT, int NBS, bool STAGED>
+void launch_regular_cells(int tier, dim3 rg, hipStream_t s, const dev::CellsArgs& A) {
+  if (HELM && tier == 2) dev::cells_regular_kernel<OP, HELM, NBT, NBS, 2, STAGED><<<rg, 256, 0, s>>>(A);
+  else if (HELM && tier == 1) dev::cells_regular_kernel<OP, HELM, NBT, NBS, 1, STAGED><<<rg, 256, 0, s>>>(A);
+  else dev::cells_regular_kernel<OP, HELM, NBT, NBS, 0, STAGED><<<rg, 256, 0, s>>>(A);
+}
+
 template <int OP, bool HELM, int NBT, int NBS>
 void launch_cells(int tier, dim3 rg, dim3 sg, hipStream_t s, const dev::CellsArgs& A) {
-  if (HELM && tier == 2) dev::cells_regular_kernel<OP, HELM, NBT, NBS, 2><<<rg, 256, 0, s>>>(A);
-  else if (HELM && tier == 1) dev::cells_regular_kernel<OP, HELM, NBT, NBS, 1><<<rg, 256, 0, s>>>(A);
-  else if (tier >= 0) dev::cells_regular_kernel<OP, HELM, NBT, NBS, 0><<<rg, 256, 0, s>>>(A);
+  if (tier >= 0) {
+    if (A.max_trial_points <= dev::kStageRecords)
+      launch_regular_cells<OP, HELM, NBT, NBS, true>(tier, rg, s, A);
+    else
+      launch_regular_cells<OP, HELM, NBT, NBS, false>(tier, rg, s, A);
+  }
   if (A.npairs > 0) dev::cells_singular_kernel<OP, HELM, NBT, NBS><<<sg, 256, 0, s>>>(A);
 }
 
@@ -519,40 +591,137 @@ double max_cell_radius(const Grid& g) {
   return rmax;
 }
 
-// The cells' point records, vertex and DOF tables and (same grid) adjacent pairs with their
-// rules, uploaded into `keep`; `A` points at them.
-void upload_cells(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test, hipStream_t s,
-                  std::vector<DeviceBuf>& keep, dev::CellsArgs& A) {
-  std::vector<double> trec, srec, tcv, scv, rule;
-  std::vector<int> tfirst, sfirst, tvid, svid, tdofs, sdofs;
-  std::vector<int4> pairs;
-  point_records(test, a.options, trec, tfirst);
-  point_records(trial, a.options, srec, sfirst);
-  cell_tables(test, tvid, tdofs, tcv);
-  cell_tables(trial, svid, sdofs, scv);
+// Device data built on first use and cached: per grid the point records for a basis degree and
+// the regular rules, the vertex tables and the adjacent pairs with their rules for the
+// singular rules; per space the DOF table. Each field is set by the key that builds it.
+struct CellsBlob : DeviceBlob {
+  std::vector<DeviceBuf> bufs;
+  const double* rec = nullptr;  // point records (dev::kRec doubles each)
+  const int* first = nullptr;   // first record of each cell [nc + 1]
+  int max_points = 0;
+  const int* vid = nullptr;     // vertex ids [nc][4]
+  const double* cv = nullptr;   // vertex coordinates [nc][4][3]
+  double max_radius = 0.0;      // largest centroid-to-vertex distance
+  const int4* pairs = nullptr;  // adjacent pairs and their rules
+  const double* rule = nullptr;
+  int npairs = 0;
+  const int* dofs = nullptr;    // DOFs [nc][4]
+
+  template <typename T>
+  const T* upload(const std::vector<T>& h) {
+    bufs.emplace_back();
+    HIP_CHECK_Q(hipMalloc(&bufs.back().p, std::max<size_t>(h.size(), 1) * sizeof(T)));
+    if (!h.empty()) HIP_CHECK_Q(hipMemcpy(bufs.back().p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return (const T*)bufs.back().p;
+  }
+};
+
+template <typename Build>
+const CellsBlob& cells_cached(std::mutex& m, DeviceBlobCache& cache, const std::string& key, Build build) {
+  int dev = 0;
+  HIP_CHECK_Q(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(m);
+  std::unique_ptr<DeviceBlob>& slot = cache[{dev, key}];
+  if (!slot) {
+    auto b = std::make_unique<CellsBlob>();
+    build(*b);
+    slot = std::move(b);
+  }
+  return static_cast<const CellsBlob&>(*slot);
+}
+
+const CellsBlob& cached_records(const FunctionSpace& sp, const Options& o) {
+  auto nq = [&](uint8_t t) {
+    auto it = o.quadrature_degrees.find(t);
+    return it == o.quadrature_degrees.end() ? -1L : (long)it->second;
+  };
+  const std::string key = "rec/" + std::to_string(sp.degree) + "/" + std::to_string(nq(kTriangle)) + "/" +
+                          std::to_string(nq(kQuadrilateral));
+  Grid& g = const_cast<Grid&>(*sp.grid);
+  return cells_cached(g.cache_mutex, g.cells_cache, key, [&](CellsBlob& b) {
+    std::vector<double> rec;
+    std::vector<int> first;
+    point_records(sp, o, rec, first);
+    b.rec = b.upload(rec);
+    b.first = b.upload(first);
+    for (size_t c = 0; c + 1 < first.size(); ++c) b.max_points = std::max(b.max_points, first[c + 1] - first[c]);
+  });
+}
+
+const CellsBlob& cached_tables(const Grid& grid) {
+  Grid& g = const_cast<Grid&>(grid);
+  return cells_cached(g.cache_mutex, g.cells_cache, "tab", [&](CellsBlob& b) {
+    std::vector<int> vid(4 * g.nc(), -1);
+    std::vector<double> cv(12 * g.nc(), 0.0);
+    for (int64_t c = 0; c < g.nc(); ++c)
+      for (int m = 0; m < g.cell_nv(c); ++m) {
+        vid[4 * c + m] = (int)g.vertex(c, m);
+        for (int d = 0; d < 3; ++d) cv[12 * c + 3 * m + d] = g.points[3 * g.vertex(c, m) + d];
+      }
+    b.vid = b.upload(vid);
+    b.cv = b.upload(cv);
+    b.max_radius = max_cell_radius(g);
+  });
+}
+
+const CellsBlob& cached_pairs(const Grid& grid, const Options& o) {
+  std::string key = "adj";
+  for (const auto& kv : o.singular_quadrature_degrees)
+    key += "/" + std::to_string(kv.first.first) + "," + std::to_string(kv.first.second) + "," +
+           std::to_string(kv.second);
+  Grid& g = const_cast<Grid&>(grid);
+  return cells_cached(g.cache_mutex, g.cells_cache, key, [&](CellsBlob& b) {
+    std::vector<int4> pairs;
+    std::vector<double> rule;
+    adjacent_cell_pairs(g, o, pairs, rule);
+    b.pairs = b.upload(pairs);
+    b.rule = b.upload(rule);
+    b.npairs = (int)pairs.size();
+  });
+}
+
+const CellsBlob& cached_dofs(const FunctionSpace& space) {
+  FunctionSpace& sp = const_cast<FunctionSpace&>(space);
+  return cells_cached(sp.cache_mutex, sp.cells_cache, "dofs", [&](CellsBlob& b) {
+    std::vector<int> dofs(4 * sp.grid->nc(), -1);
+    for (int64_t c = 0; c < sp.grid->nc(); ++c)
+      for (int k = 0; k < sp.cell_nb(c); ++k) dofs[4 * c + k] = (int)sp.dofs[(size_t)sp.nb * c + k];
+    b.dofs = b.upload(dofs);
+  });
+}
+
+// The kernels' arguments from the cached device data (no allocation once cached).
+void cells_args(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test, dev::CellsArgs& A) {
+  const CellsBlob& tr = cached_records(test, a.options);
+  const CellsBlob& sr = cached_records(trial, a.options);
+  const CellsBlob& tt = cached_tables(*test.grid);
+  const CellsBlob& st = cached_tables(*trial.grid);
   const bool same = test.grid == trial.grid;
-  if (same) adjacent_cell_pairs(*test.grid, a.options, pairs, rule);
-  keep.reserve(16);
   A = dev::CellsArgs{};
-  A.trec = upload_to(keep, trec, s);
-  A.srec = upload_to(keep, srec, s);
-  A.tfirst = upload_to(keep, tfirst, s);
-  A.sfirst = upload_to(keep, sfirst, s);
-  A.tvid = upload_to(keep, tvid, s);
-  A.svid = upload_to(keep, svid, s);
-  A.tdofs = upload_to(keep, tdofs, s);
-  A.sdofs = upload_to(keep, sdofs, s);
-  A.tcv = upload_to(keep, tcv, s);
-  A.scv = upload_to(keep, scv, s);
-  A.rule = upload_to(keep, rule, s);
-  A.pairs = upload_to(keep, pairs, s);
-  A.npairs = (int)pairs.size();
+  A.trec = tr.rec;
+  A.srec = sr.rec;
+  A.tfirst = tr.first;
+  A.sfirst = sr.first;
+  A.tvid = tt.vid;
+  A.svid = st.vid;
+  A.tcv = tt.cv;
+  A.scv = st.cv;
+  A.tdofs = cached_dofs(test).dofs;
+  A.sdofs = cached_dofs(trial).dofs;
+  if (same) {
+    const CellsBlob& pr = cached_pairs(*test.grid, a.options);
+    A.pairs = pr.pairs;
+    A.rule = pr.rule;
+    A.npairs = pr.npairs;
+  }
   A.n_test = (int)test.grid->nc();
   A.n_trial = (int)trial.grid->nc();
   A.same_grid = same ? 1 : 0;
   A.k = a.k;
   A.ld = (long long)test.size;
   A.out_type = cells_out_type(a.dtype);
+  A.max_trial_points = sr.max_points;
+  A.radius_sum = tt.max_radius + st.max_radius;
 }
 
 }  // namespace
@@ -568,14 +737,13 @@ void assemble_cells_singular_csr(const Assembler& a, const FunctionSpace& trial,
   out.indices.clear();
   out.data.clear();
   if (test.grid != trial.grid || test.grid->nc() == 0) return;  // no neighbouring cells across grids
-  std::vector<DeviceBuf> keep;
   dev::CellsArgs A;
-  upload_cells(a, trial, test, s, keep, A);
+  cells_args(a, trial, test, A);
   if (A.npairs == 0) return;
   const int nbt = test.degree == 0 ? 1 : 4, nbs = trial.degree == 0 ? 1 : 4;
-  keep.emplace_back();
-  HIP_CHECK_Q(hipMalloc(&keep.back().p, (size_t)A.npairs * nbt * nbs * 2 * sizeof(double)));
-  A.coo = (double*)keep.back().p;
+  DeviceBuf coo;
+  HIP_CHECK_Q(hipMalloc(&coo.p, (size_t)A.npairs * nbt * nbs * 2 * sizeof(double)));
+  A.coo = (double*)coo.p;
   A.correction = correction ? 1 : 0;
   const dim3 sg((unsigned)((A.npairs + 3) / 4));
   if (a.kind == kHelmholtz)
@@ -584,7 +752,7 @@ void assemble_cells_singular_csr(const Assembler& a, const FunctionSpace& trial,
     launch_cells_op<false>(a.op, nbt, nbs, -1, dim3(1), sg, s, A);
   HIP_CHECK_Q(hipGetLastError());
   coo_to_csr_device(A.coo, A.pairs, A.npairs, A.tdofs, nbt, 4, A.sdofs, nbs, 4, test.size, trial.size, a.dtype, s, out);
-  // coo_to_csr_device ends with a stream synchronisation: the uploads may be freed
+  // coo_to_csr_device ends with a stream synchronisation: the COO buffer may be freed
 }
 
 void assemble_cells_device(const Assembler& a, const FunctionSpace& trial, const FunctionSpace& test, void* dev_out,
@@ -599,9 +767,8 @@ void assemble_cells_device(const Assembler& a, const FunctionSpace& trial, const
     HIP_CHECK_Q(hipGetLastError());
   }
   if (test.grid->nc() == 0 || trial.grid->nc() == 0) return;
-  std::vector<DeviceBuf> keep;
   dev::CellsArgs A;
-  upload_cells(a, trial, test, s, keep, A);
+  cells_args(a, trial, test, A);
   A.out = dev_out;
   const int bx = (A.n_test + 255) / 256;
   long long chunk = ((long long)bx * A.n_trial + 16383) / 16384;
@@ -615,7 +782,7 @@ void assemble_cells_device(const Assembler& a, const FunctionSpace& trial, const
   // margin for rounding (the tiers' polynomials hold to 0.5 and 0.25 exactly)
   int tier = 0;
   if (a.kind == kHelmholtz && a.k != 0.0) {
-    const double bound = std::fabs(a.k) * (max_cell_radius(*test.grid) + max_cell_radius(*trial.grid)) * (1.0 + 1e-12);
+    const double bound = std::fabs(a.k) * A.radius_sum * (1.0 + 1e-12);
     tier = bound <= 0.25 ? 2 : (bound <= 0.5 ? 1 : 0);
   }
   if (a.kind == kHelmholtz)
@@ -623,8 +790,6 @@ void assemble_cells_device(const Assembler& a, const FunctionSpace& trial, const
   else
     launch_cells_op<false>(a.op, nbt, nbs, 0, rg, sg, s, A);
   HIP_CHECK_Q(hipGetLastError());
-  // the uploads above are freed when `keep` goes out of scope: wait for the kernels
-  HIP_CHECK_Q(hipStreamSynchronize(s));
 }
 
 }  // namespace bempp_amd

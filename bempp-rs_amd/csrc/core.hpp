@@ -52,6 +52,13 @@ struct DeviceSpaceCache;  // device.cpp
 void free_device_grid_cache(DeviceGridCache*);
 void free_device_space_cache(DeviceSpaceCache*);
 
+// Device data of the kernels for grids with quadrilaterals, cached per grid or space and device
+// under a key naming what it was built from; freed with its owner.
+struct DeviceBlob {
+  virtual ~DeviceBlob() = default;
+};
+using DeviceBlobCache = std::map<std::pair<int, std::string>, std::unique_ptr<DeviceBlob>>;
+
 // ---------------------------------------------------------------------------
 // Grid of flat triangles: vertex coordinates and cell -> vertex ids, plus the
 // vertex -> cells adjacency the singular pair classification walks
@@ -81,6 +88,7 @@ struct Grid {
 
   std::mutex cache_mutex;
   std::map<int, DeviceGridCache*> device_cache;  // device id -> cache
+  DeviceBlobCache cells_cache;                   // grids with quadrilaterals (cells.hip)
 };
 
 Grid* make_sphere(int seed, uint32_t level);
@@ -105,6 +113,7 @@ struct FunctionSpace {
 
   std::mutex cache_mutex;
   std::map<int, DeviceSpaceCache*> device_cache;
+  DeviceBlobCache cells_cache;  // grids with quadrilaterals (cells.hip)
 };
 
 // ---------------------------------------------------------------------------

@@ -137,6 +137,9 @@ __global__ void __launch_bounds__(256) csr_indptr_kernel(const long long* __rest
 
 struct DeviceBuffers {
   void* p = nullptr;
+  DeviceBuffers() = default;
+  DeviceBuffers(const DeviceBuffers&) = delete;
+  DeviceBuffers& operator=(const DeviceBuffers&) = delete;
   ~DeviceBuffers() {
     if (p) (void)hipFree(p);
   }

@@ -93,6 +93,7 @@ void Grid::set_points(const double* pts) {
   ++version;
   for (auto& kv : device_cache) free_device_grid_cache(kv.second);
   device_cache.clear();
+  cells_cache.clear();
 }
 
 // Sphere by repeated midpoint refinement with projection to radius 1

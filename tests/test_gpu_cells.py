@@ -211,3 +211,29 @@ def test_quadrilateral_singular_csr(mesh, kind, k, op):
             err = np.linalg.norm(csr.toarray() - R) / np.linalg.norm(R)
             print(f"[cells-csr]   nnz {csr.nnz}, rel err {err:.2e}", flush=True)
             assert err <= 1e-12
+
+
+def test_cached_device_data_follows_rules_and_points():
+    """The general-cell device data (point records, tables, adjacent pairs, DOFs) is cached per
+    grid and space: on one grid object, two regular rules each match the oracle, and after
+    set_points the assembly follows the moved points."""
+    pts, cells = cubed_sphere(4)
+    g = Grid(pts, cells)
+    s = _space(g, "p1")
+    for nq in (16, 9, 16):
+        A = _assembler("helmholtz", "double_layer", 2.0, _options(nq_quad=nq)).assemble(s, s)
+        R = O.assemble_cells("helmholtz", "double_layer", pts, cells, test=(1, True), trial=(1, True), k=2.0,
+                             nq_quad=nq)
+        assert np.linalg.norm(A - R) / np.linalg.norm(R) <= 1e-12, nq
+    moved = pts * np.array([1.0, 1.2, 0.9])
+    g.set_points(moved)
+    a = _assembler("helmholtz", "double_layer", 2.0, _options())
+    A = a.assemble(s, s)
+    R = O.assemble_cells("helmholtz", "double_layer", moved, cells, test=(1, True), trial=(1, True), k=2.0)
+    err = np.linalg.norm(A - R) / np.linalg.norm(R)
+    print(f"[cells] after set_points: rel err {err:.2e}", flush=True)
+    assert err <= 1e-12
+    S = a.assemble_singular(s, s).toarray()
+    RS = O.assemble_cells("helmholtz", "double_layer", moved, cells, test=(1, True), trial=(1, True), k=2.0,
+                          part="singular")
+    assert np.linalg.norm(S - RS) / np.linalg.norm(RS) <= 1e-12
