@@ -63,6 +63,7 @@ struct CellsArgs {
   int npairs;
   int correction;      // adjacent pairs under the cells' regular rules instead of their pair rules
   double* coo;         // non-null: the adjacent pairs' values in COO layout instead of `out`
+  int symmetric;        // single layer / hypersingular on one space: pairs j > i only, mirrored
   int max_trial_points;  // host: the most regular points of a trial cell (picks the staged kernel)
   double radius_sum;     // host: largest test plus largest trial centroid-to-vertex distance
 };
@@ -192,8 +193,12 @@ __global__ void __launch_bounds__(256) cells_regular_kernel(const CellsArgs A) {
   const auto sfirst = constant_view(A.sfirst);
   const int j0 = blockIdx.y * A.trial_chunk;
   const int j1 = min(j0 + A.trial_chunk, A.n_trial);
+  // symmetric: a block whose trial cells all precede its test cells has nothing to do
+  // (uniform over the block, before any barrier)
+  if (A.symmetric && j1 <= (int)(blockIdx.x * blockDim.x)) return;
   // trial cell j, its nq point records from Qj on
   auto trial_cell = [&](int j, const auto* Qj, int nq) {
+    if (A.symmetric && j <= i) return;  // j < i: the mirror of pair (j, i); j = i: adjacent
     bool adjacent = false;
     if (A.same_grid)
 #pragma unroll
@@ -300,9 +305,12 @@ __global__ void __launch_bounds__(256) cells_regular_kernel(const CellsArgs A) {
       if (col < 0) continue;  // uniform: the trial cell's basis count
 #pragma unroll
       for (int a = 0; a < NBT; ++a)
-        if (td[a] >= 0)
-          store_value(A.out, (long long)td[a] + A.ld * col, {M[a][b].re * kInv4Pi, M[a][b].im * kInv4Pi}, A.out_type,
-                      kAtomic);
+        if (td[a] >= 0) {
+          const cplx v = {M[a][b].re * kInv4Pi, M[a][b].im * kInv4Pi};
+          store_value(A.out, (long long)td[a] + A.ld * col, v, A.out_type, kAtomic);
+          // the transposed pair (j, i) adds the same block transposed (symmetric kernel, one space)
+          if (A.symmetric) store_value(A.out, (long long)col + A.ld * td[a], v, A.out_type, kAtomic);
+        }
     }
   };
   if constexpr (STAGED) {
@@ -770,6 +778,9 @@ void assemble_cells_device(const Assembler& a, const FunctionSpace& trial, const
   dev::CellsArgs A;
   cells_args(a, trial, test, A);
   A.out = dev_out;
+  // the single-layer and hypersingular kernels are symmetric in (x, y): on one space the pair
+  // (j, i) gives the transpose of the block of (i, j)
+  A.symmetric = (a.op == kSL || a.op == kHS) && &test == &trial ? 1 : 0;
   const int bx = (A.n_test + 255) / 256;
   long long chunk = ((long long)bx * A.n_trial + 16383) / 16384;
   chunk = std::max(4LL, std::min(64LL, chunk));
