@@ -44,6 +44,7 @@ constexpr int kRec = 24;  // doubles per point record
 
 struct CellsArgs {
   const double* trec;  // test point records [points][kRec]
+  const double* txw;   // the same points' (x, y, z, w J) [points][4] (the fields every operator reads)
   const double* srec;
   const int* tfirst;   // first record of each test cell [n_test + 1]
   const int* sfirst;
@@ -64,6 +65,7 @@ struct CellsArgs {
   int correction;      // adjacent pairs under the cells' regular rules instead of their pair rules
   double* coo;         // non-null: the adjacent pairs' values in COO layout instead of `out`
   int symmetric;        // single layer / hypersingular on one space: pairs j > i only, mirrored
+  int store_mode;       // regular entries: kStore / kAccumulate (one writer each) or kAtomic
   int max_trial_points;  // host: the most regular points of a trial cell (picks the staged kernel)
   double radius_sum;     // host: largest test plus largest trial centroid-to-vertex distance
 };
@@ -229,12 +231,13 @@ __global__ void __launch_bounds__(256) cells_regular_kernel(const CellsArgs A) {
 #pragma unroll
       for (int u = 0; u < PP; ++u) {
         const double* P = A.trec + (size_t)(p + u) * kRec;
+        const double* XW = A.txw + 4 * (size_t)(p + u);  // re-read per trial cell: 32 B, not a record
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          xp[u][d] = P[d];
+          xp[u][d] = XW[d];
           np[u][d] = P[4 + d];
         }
-        wJp[u] = P[3];
+        wJp[u] = XW[3];
         wp[u] = P[7];
 #pragma unroll
         for (int a = 0; a < NBT; ++a) {
@@ -268,8 +271,9 @@ __global__ void __launch_bounds__(256) cells_regular_kernel(const CellsArgs A) {
     } else {
       for (int p = p0; p < p1; ++p) {
         const double* P = A.trec + (size_t)p * kRec;
-        const double xp[3] = {P[0], P[1], P[2]}, np[3] = {P[4], P[5], P[6]};
-        const double wJp = P[3], wp = P[7];
+        const double* XW = A.txw + 4 * (size_t)p;
+        const double xp[3] = {XW[0], XW[1], XW[2]}, np[3] = {P[4], P[5], P[6]};
+        const double wJp = XW[3], wp = P[7];
         double php[NBT], cp[NBT][3];
 #pragma unroll
         for (int a = 0; a < NBT; ++a) {
@@ -307,9 +311,9 @@ __global__ void __launch_bounds__(256) cells_regular_kernel(const CellsArgs A) {
       for (int a = 0; a < NBT; ++a)
         if (td[a] >= 0) {
           const cplx v = {M[a][b].re * kInv4Pi, M[a][b].im * kInv4Pi};
-          store_value(A.out, (long long)td[a] + A.ld * col, v, A.out_type, kAtomic);
+          store_value(A.out, (long long)td[a] + A.ld * col, v, A.out_type, A.store_mode);
           // the transposed pair (j, i) adds the same block transposed (symmetric kernel, one space)
-          if (A.symmetric) store_value(A.out, (long long)col + A.ld * td[a], v, A.out_type, kAtomic);
+          if (A.symmetric) store_value(A.out, (long long)col + A.ld * td[a], v, A.out_type, A.store_mode);
         }
     }
   };
@@ -605,6 +609,7 @@ double max_cell_radius(const Grid& g) {
 struct CellsBlob : DeviceBlob {
   std::vector<DeviceBuf> bufs;
   const double* rec = nullptr;  // point records (dev::kRec doubles each)
+  const double* xw = nullptr;   // their (x, y, z, w J), 4 doubles each
   const int* first = nullptr;   // first record of each cell [nc + 1]
   int max_points = 0;
   const int* vid = nullptr;     // vertex ids [nc][4]
@@ -651,6 +656,10 @@ const CellsBlob& cached_records(const FunctionSpace& sp, const Options& o) {
     std::vector<int> first;
     point_records(sp, o, rec, first);
     b.rec = b.upload(rec);
+    std::vector<double> xw(4 * (rec.size() / dev::kRec));
+    for (size_t r = 0; r < xw.size() / 4; ++r)
+      for (int m = 0; m < 4; ++m) xw[4 * r + m] = rec[dev::kRec * r + m];
+    b.xw = b.upload(xw);
     b.first = b.upload(first);
     for (size_t c = 0; c + 1 < first.size(); ++c) b.max_points = std::max(b.max_points, first[c + 1] - first[c]);
   });
@@ -707,6 +716,7 @@ void cells_args(const Assembler& a, const FunctionSpace& trial, const FunctionSp
   const bool same = test.grid == trial.grid;
   A = dev::CellsArgs{};
   A.trec = tr.rec;
+  A.txw = tr.xw;
   A.srec = sr.rec;
   A.tfirst = tr.first;
   A.sfirst = sr.first;
@@ -781,6 +791,11 @@ void assemble_cells_device(const Assembler& a, const FunctionSpace& trial, const
   // the single-layer and hypersingular kernels are symmetric in (x, y): on one space the pair
   // (j, i) gives the transpose of the block of (i, j)
   A.symmetric = (a.op == kSL || a.op == kHS) && &test == &trial ? 1 : 0;
+  // without DOFs shared between cells (P0, discontinuous P1 on both sides) every regular entry
+  // has one writer (adjacent entries belong to the pair kernel): plain stores after the zero
+  // fill, or a read-add-write when accumulating, instead of atomics
+  const bool one_writer = (test.degree == 0 || !test.continuous) && (trial.degree == 0 || !trial.continuous);
+  A.store_mode = one_writer ? (accumulate ? dev::kAccumulate : dev::kStore) : dev::kAtomic;
   const int bx = (A.n_test + 255) / 256;
   long long chunk = ((long long)bx * A.n_trial + 16383) / 16384;
   chunk = std::max(4LL, std::min(64LL, chunk));

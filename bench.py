@@ -425,9 +425,17 @@ def main():
                     help="with --configs: comma-separated substrings of '<config>_<precision>' to run")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB copy used to calibrate the HBM byte counters")
+    ap.add_argument("--quadrilateral", action="store_true",
+                    help="only the quadrilateral-grid leg (9600-cell cubed sphere), one JSON line")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.quadrilateral:
+        import torch
+
+        torch.cuda.set_device(0)
+        print(json.dumps(quadrilateral_leg(40, args.k, max(1, args.steps))))
         return
     if args.configs:
         config_suite(max(1, min(args.steps, 5)), max(1, min(args.warmup, 3)), args.only)

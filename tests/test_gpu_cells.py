@@ -237,3 +237,26 @@ def test_cached_device_data_follows_rules_and_points():
     RS = O.assemble_cells("helmholtz", "double_layer", moved, cells, test=(1, True), trial=(1, True), k=2.0,
                           part="singular")
     assert np.linalg.norm(S - RS) / np.linalg.norm(RS) <= 1e-12
+
+
+@pytest.mark.parametrize("sp", ["dp0", "dp1"])
+@pytest.mark.parametrize("op", ["single_layer", "double_layer"])
+def test_quadrilateral_device_overwrite_and_accumulate(sp, op):
+    """Spaces without DOFs shared between cells: every regular entry has one writer, stored
+    without atomics — overwriting a device buffer that held NaN, then accumulating onto it;
+    symmetric (single layer) and non-symmetric (double layer) assembly."""
+    import torch
+
+    pts, cells = _mesh("mixed_cube")
+    g = Grid(pts, cells)
+    s = _space(g, sp)
+    a = _assembler("helmholtz", op, 2.0, _options())
+    R = O.assemble_cells("helmholtz", op, pts, cells, test=SPACES[sp], trial=SPACES[sp], k=2.0)
+    n = s.global_size
+    D = torch.full((n, n), float("nan"), dtype=torch.complex128, device="cuda").t()
+    a.assemble_into_device(s, s, D)
+    torch.cuda.synchronize()
+    assert np.linalg.norm(D.cpu().numpy() - R) / np.linalg.norm(R) <= 1e-12
+    a.assemble_into_device(s, s, D, accumulate=True)
+    torch.cuda.synchronize()
+    assert np.linalg.norm(D.cpu().numpy() - 2 * R) / np.linalg.norm(R) <= 1e-12
