@@ -198,6 +198,7 @@ __global__ void __launch_bounds__(256) cells_regular_kernel(const CellsArgs A) {
   // symmetric: a block whose trial cells all precede its test cells has nothing to do
   // (uniform over the block, before any barrier)
   if (A.symmetric && j1 <= (int)(blockIdx.x * blockDim.x)) return;
+  constexpr bool kScaledSL = OP == 0 && HELM && TIER > 0 && NBT == 1 && NBS == 1;
   // trial cell j, its nq point records from Qj on
   auto trial_cell = [&](int j, const auto* Qj, int nq) {
     if (A.symmetric && j <= i) return;  // j < i: the mirror of pair (j, i); j = i: adjacent
@@ -247,21 +248,52 @@ __global__ void __launch_bounds__(256) cells_regular_kernel(const CellsArgs A) {
             for (int d = 0; d < 3; ++d) cp[u][a][d] = P[12 + 3 * a + d];
         }
       }
-      for (int q = 0; q < nq; ++q) {
-        const auto Q = Qj + (size_t)q * kRec;
-        const double xq[3] = {Q[0], Q[1], Q[2]}, nq3[3] = {Q[4], Q[5], Q[6]};
-        double phq[NBS], cq[NBS][3];
+      if constexpr (kScaledSL) {
+        // P0 single layer, centred tiers: lengths scaled by |k| / sqrt 2 (h = k^2 r^2 / 2), one
+        // Newton step y (3 - h y^2) of y = rsq(h) giving rz = 2 sqrt(2) / (|k| r) and the phase
+        // h rz - sqrt(2) |k| R in one FMA; the trial weight with rz, the test weight per point
+        const double sc = 0.5 * A.k * A.k, kRs = 1.4142135623730951 * fabs(kR);
+        cplx ap[PP];
 #pragma unroll
-        for (int b = 0; b < NBS; ++b) {
-          phq[b] = NBS == 1 ? 1.0 : Q[8 + b];
-          if constexpr (OP == 3)
+        for (int u = 0; u < PP; ++u) ap[u] = {0.0, 0.0};
+        for (int q = 0; q < nq; ++q) {
+          const auto Q = Qj + (size_t)q * kRec;
+          const double xq[3] = {Q[0], Q[1], Q[2]}, wq = Q[3];
 #pragma unroll
-            for (int d = 0; d < 3; ++d) cq[b][d] = Q[12 + 3 * b + d];
+          for (int u = 0; u < PP; ++u) {
+            const double dx = xp[u][0] - xq[0], dy = xp[u][1] - xq[1], dz = xp[u][2] - xq[2];
+            const double h = sc * (dx * dx + dy * dy + dz * dz);
+            const double y = __builtin_amdgcn_rsq(h);
+            const double rz = y * fma(-(h * y), y, 3.0);
+            double sn, cs;
+            sincos_small<TIER, true>(fma(h, rz, -kRs), &sn, &cs);
+            const double w = rz * wq;
+            ap[u].re = fma(cs, w, ap[u].re);
+            ap[u].im = fma(sn, w, ap[u].im);
+          }
         }
 #pragma unroll
-        for (int u = 0; u < PP; ++u)
-          add_point_pair<OP, HELM, NBT, NBS, TIER>(xp[u], np[u], php[u], cp[u], xq, nq3, phq, cq, wJp[u] * Q[3],
-                                                   wp[u] * Q[7], A.k, M, kR);
+        for (int u = 0; u < PP; ++u) {
+          M[0][0].re = fma(ap[u].re, wJp[u], M[0][0].re);
+          M[0][0].im = fma(ap[u].im, wJp[u], M[0][0].im);
+        }
+      } else {
+        for (int q = 0; q < nq; ++q) {
+          const auto Q = Qj + (size_t)q * kRec;
+          const double xq[3] = {Q[0], Q[1], Q[2]}, nq3[3] = {Q[4], Q[5], Q[6]};
+          double phq[NBS], cq[NBS][3];
+#pragma unroll
+          for (int b = 0; b < NBS; ++b) {
+            phq[b] = NBS == 1 ? 1.0 : Q[8 + b];
+            if constexpr (OP == 3)
+#pragma unroll
+              for (int d = 0; d < 3; ++d) cq[b][d] = Q[12 + 3 * b + d];
+          }
+#pragma unroll
+          for (int u = 0; u < PP; ++u)
+            add_point_pair<OP, HELM, NBT, NBS, TIER>(xp[u], np[u], php[u], cp[u], xq, nq3, phq, cq, wJp[u] * Q[3],
+                                                     wp[u] * Q[7], A.k, M, kR);
+        }
       }
     };
     if constexpr (NBT * NBS <= 4) {
@@ -297,6 +329,12 @@ __global__ void __launch_bounds__(256) cells_regular_kernel(const CellsArgs A) {
                                                    kR);
         }
       }
+    }
+    if constexpr (kScaledSL) {
+      // the sums hold 2 sqrt(2) cos(a) / (|k| r) and 4 sin(a) / (|k| r), a = |k| (r - R);
+      // sin(k (r - R)) = sign(k) sin a
+      M[0][0].re *= fabs(A.k) * (0.25 * 1.4142135623730951);
+      M[0][0].im *= 0.25 * A.k;
     }
     if constexpr (TIER > 0)
 #pragma unroll

@@ -123,7 +123,7 @@ def test_quadrilateral_device_output_dtypes_and_modes():
     assert l32.dtype == np.float32 and np.linalg.norm(l32 - L) / np.linalg.norm(L) <= 1e-6
 
 
-@pytest.mark.parametrize("k,tier", [(1.2, 1), (0.6, 2)])
+@pytest.mark.parametrize("k,tier", [(1.2, 1), (0.6, 2), (-1.2, 1), (-0.6, 2)])
 @pytest.mark.parametrize("op", list(OPS))
 def test_quadrilateral_centred_phase_tiers(k, tier, op):
     """Small |k| x cell size on a 384-cell cubed sphere: the regular kernel takes the phase
