@@ -67,6 +67,7 @@ struct CellsArgs {
   int symmetric;        // single layer / hypersingular on one space: pairs j > i only, mirrored
   int store_mode;       // regular entries: kStore / kAccumulate (one writer each) or kAtomic
   int max_trial_points;  // host: the most regular points of a trial cell (picks the staged kernel)
+  int adjacent_tier;     // host: phase tier of the adjacent pairs (picks the pair kernel)
   double radius_sum;     // host: largest test plus largest trial centroid-to-vertex distance
 };
 
@@ -91,8 +92,9 @@ __device__ __forceinline__ void cell_point(const double* V, bool quad, double x,
     }
   }
   const double c[3] = {tx[1] * ty[2] - tx[2] * ty[1], tx[2] * ty[0] - tx[0] * ty[2], tx[0] * ty[1] - tx[1] * ty[0]};
-  J = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
-  const double ji = 1.0 / J;
+  const double c2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+  const double ji = rsqrt_fast(c2);  // 1 / J and J = c2 / J without an IEEE square root or division
+  J = c2 * ji;
   n[0] = c[0] * ji, n[1] = c[1] * ji, n[2] = c[2] * ji;
   if constexpr (NB == 1) {
     phi[0] = 1.0;
@@ -374,7 +376,9 @@ __global__ void __launch_bounds__(256) cells_regular_kernel(const CellsArgs A) {
   }
 }
 
-template <int OP, bool HELM, int NBT, int NBS>
+// TIER (Helmholtz): 0 full argument reduction; 1, 2 the small-argument sine/cosine of k r
+// directly, |k r| <= 0.5 / 0.25 for every point pair of two cells sharing a vertex.
+template <int OP, bool HELM, int NBT, int NBS, int TIER = 0>
 __global__ void __launch_bounds__(256) cells_singular_kernel(const CellsArgs A) {
   const int pair = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6));
   if (pair >= A.npairs) return;  // uniform per wavefront
@@ -419,7 +423,7 @@ __global__ void __launch_bounds__(256) cells_singular_kernel(const CellsArgs A) 
 #pragma unroll
         for (int d = 0; d < 3; ++d) cq[b][d] = Q[12 + 3 * b + d];
       }
-      add_point_pair<OP, HELM, NBT, NBS>(xp, np, php, cp, xq, nq, phq, cq, P[3] * Q[3], P[7] * Q[7], A.k, M);
+      add_point_pair<OP, HELM, NBT, NBS, TIER>(xp, np, php, cp, xq, nq, phq, cq, P[3] * Q[3], P[7] * Q[7], A.k, M);
     }
   } else {
     for (int i = lane; i < pr.w; i += 64) {
@@ -427,7 +431,7 @@ __global__ void __launch_bounds__(256) cells_singular_kernel(const CellsArgs A) 
       double xp[3], np[3], Jp, php[4], cp[4][3], xq[3], nq[3], Jq, phq[4], cq[4][3];
       cell_point<NBT>(tV, tq, r[0], r[1], xp, np, Jp, php, cp);
       cell_point<NBS>(sV, sq, r[2], r[3], xq, nq, Jq, phq, cq);
-      add_point_pair<OP, HELM, NBT, NBS>(xp, np, php, cp, xq, nq, phq, cq, r[4] * Jp * Jq, r[4], A.k, M);
+      add_point_pair<OP, HELM, NBT, NBS, TIER>(xp, np, php, cp, xq, nq, phq, cq, r[4] * Jp * Jq, r[4], A.k, M);
     }
   }
 #pragma unroll
@@ -598,7 +602,11 @@ void launch_cells(int tier, dim3 rg, dim3 sg, hipStream_t s, const dev::CellsArg
     else
       launch_regular_cells<OP, HELM, NBT, NBS, false>(tier, rg, s, A);
   }
-  if (A.npairs > 0) dev::cells_singular_kernel<OP, HELM, NBT, NBS><<<sg, 256, 0, s>>>(A);
+  if (A.npairs > 0) {
+    if (HELM && A.adjacent_tier == 2) dev::cells_singular_kernel<OP, HELM, NBT, NBS, 2><<<sg, 256, 0, s>>>(A);
+    else if (HELM && A.adjacent_tier == 1) dev::cells_singular_kernel<OP, HELM, NBT, NBS, 1><<<sg, 256, 0, s>>>(A);
+    else dev::cells_singular_kernel<OP, HELM, NBT, NBS, 0><<<sg, 256, 0, s>>>(A);
+  }
 }
 
 template <int OP, bool HELM>
@@ -778,6 +786,13 @@ void cells_args(const Assembler& a, const FunctionSpace& trial, const FunctionSp
   A.out_type = cells_out_type(a.dtype);
   A.max_trial_points = sr.max_points;
   A.radius_sum = tt.max_radius + st.max_radius;
+  // points of two cells sharing a vertex are at most 2 (rad_i + rad_j) apart: |k r| <= |k| 2 A.radius_sum
+  // (with a margin for rounding; the tiers' polynomials hold to 0.5 and 0.25 exactly)
+  A.adjacent_tier = 0;
+  if (a.kind == kHelmholtz && same) {
+    const double bound = std::fabs(a.k) * 2.0 * A.radius_sum * (1.0 + 1e-12);
+    A.adjacent_tier = bound <= 0.25 ? 2 : (bound <= 0.5 ? 1 : 0);
+  }
 }
 
 }  // namespace
