@@ -784,12 +784,23 @@ __device__ __forceinline__ cplx p0_pair_orbit(const double (&td)[2][3], const do
 template <typename Args>
 __device__ __forceinline__ void flush_tile(const Args& A, const double2* acc, int t0, int ndt, int s0, int nds,
                                            bool mirror) {
+  // the blocks' DOF numbers copied to LDS first (at most 256 a side; larger blocks read them
+  // from global memory): each entry's row and column then cost an LDS read, not a global load
+  // and its wait
+  __shared__ int frow[256], fcol[256];
+  const bool staged = ndt <= 256 && nds <= 256;  // block-uniform
+  if (staged) {
+    for (int t = threadIdx.x; t < ndt; t += blockDim.x) frow[t] = A.test_blk_dofs[t0 + t];
+    for (int t = threadIdx.x; t < nds; t += blockDim.x) fcol[t] = A.trial_blk_dofs[s0 + t];
+    __syncthreads();
+  }
   const int nacc = ndt * nds;
   for (int e = threadIdx.x; e < nacc; e += blockDim.x) {
     const double2 v = acc[e];
     if (v.x == 0.0 && v.y == 0.0) continue;
     const int lsi = e / ndt, lti = e - lsi * ndt;
-    const long long row = A.test_blk_dofs[t0 + lti], col = A.trial_blk_dofs[s0 + lsi];
+    const long long row = staged ? frow[lti] : A.test_blk_dofs[t0 + lti];
+    const long long col = staged ? fcol[lsi] : A.trial_blk_dofs[s0 + lsi];
     store_value(A.out, row + A.ld * col, {v.x, v.y}, A.out_type, kAtomic);
   }
   if (!mirror) return;
@@ -797,7 +808,8 @@ __device__ __forceinline__ void flush_tile(const Args& A, const double2* acc, in
     const int lti = e / nds, lsi = e - lti * nds;
     const double2 v = acc[lsi * ndt + lti];
     if (v.x == 0.0 && v.y == 0.0) continue;
-    const long long row = A.test_blk_dofs[t0 + lti], col = A.trial_blk_dofs[s0 + lsi];
+    const long long row = staged ? frow[lti] : A.test_blk_dofs[t0 + lti];
+    const long long col = staged ? fcol[lsi] : A.trial_blk_dofs[s0 + lsi];
     store_value(A.out, col + A.ld * row, {v.x, v.y}, A.out_type, kAtomic);
   }
 }
