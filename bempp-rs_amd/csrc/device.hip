@@ -976,7 +976,8 @@ void assemble_device(const Assembler& a, const FunctionSpace& trial, const Funct
       const DeviceSpaceCache* tb = space_blocks(const_cast<FunctionSpace&>(test), P.device);
       const DeviceSpaceCache* sb = space_blocks(const_cast<FunctionSpace&>(trial), P.device);
       const size_t lds = (size_t)tb->max_nd * (size_t)sb->max_nd * sizeof(double2);
-      if (lds <= kMaxTileLds) {
+      // + the flush's static copies of the blocks' DOF numbers (2 x 256 ints)
+      if (lds + 2 * 256 * sizeof(int) <= kMaxTileLds) {
         R.test_blk_off = tb->blk_off;
         R.test_blk_dofs = tb->blk_dofs;
         R.test_ldof = tb->ldof;
