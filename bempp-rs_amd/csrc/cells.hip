@@ -12,11 +12,14 @@
 // every half pair shares the pair's common vertex, each half pair taking the triangle
 // Sauter-Schwab rule through the composite map.
 //
-// Regular pairs: one thread per test cell, trial cells wave-uniform (scalar loads); each
-// cell's points are precomputed records (point, w J, normal, w, basis values, J-scaled
-// surface curls t_eta d_xi(phi) - t_xi d_eta(phi)). Adjacent pairs: one wavefront per pair,
-// the geometry evaluated at each rule point, wavefront reductions. Both add into the output
-// with atomics.
+// Regular pairs: one thread per test cell; each cell's points are precomputed records (point,
+// w J, normal, w, basis values, J-scaled surface curls t_eta d_xi(phi) - t_xi d_eta(phi)); the
+// workgroup's trial cells are staged in LDS and read back as broadcasts; symmetric operators
+// on one space evaluate pairs j > i and store each block also transposed; entries with one
+// writer (no DOFs shared between cells) are stored plainly, the others with atomics.
+// Adjacent pairs: one wavefront per pair, the geometry evaluated at each rule point,
+// wavefront reductions, atomics. The records, tables and pair rules are built on the host on
+// first use and cached on the device with the grid and spaces.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
