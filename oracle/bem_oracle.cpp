@@ -1259,14 +1259,19 @@ bool cells_touch(const CGrid& g, int64_t a, int64_t b) {
 // dense test x trial matrix (column-major) = regular part + singular part
 // part: 0 both parts, 1 the regular (non-adjacent) pairs only, 2 the adjacent pairs only,
 // 3 the adjacent pairs under the regular rule (the singular correction)
+// rows r0..r1-1 only (r1 > r0; a P0 test space, whose DOF c is cell c): out is (r1 - r0) x trial
 template <typename T>
 void assemble_cells_t(int kind, int op, double k, const CGrid& g, const CSpace& test, const CSpace& trial, int nq_tri,
-                      int nq_quad, int nsing, int nthreads, int part, T* out) {
+                      int nq_quad, int nsing, int nthreads, int part, int64_t r0, int64_t r1, T* out) {
+  const bool restricted = r1 > r0;
+  if (restricted && (test.degree != 0 || r0 < 0 || r1 > g.nc))
+    throw std::runtime_error("a row range needs a P0 test space and rows within the grid's cells");
   const bool helm = kind == 1;
   std::vector<double> rp[2], rw[2];
   cell_regular_rule(3, nq_tri, rp[0], rw[0]);
   cell_regular_rule(4, nq_quad, rp[1], rw[1]);
-  const int64_t ld = test.size;
+  const int64_t ld = restricted ? r1 - r0 : test.size, row0 = restricted ? r0 : 0;
+  const int64_t tc_begin = restricted ? r0 : 0, tc_end = restricted ? r1 : g.nc;
   std::vector<std::vector<CPoint>> tpts(g.nc), spts(g.nc);
   for (int64_t c = 0; c < g.nc; ++c) {
     const int r = g.nvc[c] == 3 ? 0 : 1;
@@ -1280,7 +1285,7 @@ void assemble_cells_t(int kind, int op, double k, const CGrid& g, const CSpace& 
   const int nt = std::max(1, nthreads);
   std::vector<std::vector<T>> partial(nt, std::vector<T>((size_t)ld * trial.size, T(0.0)));
 #pragma omp parallel for schedule(dynamic, 4) num_threads(nt)
-  for (int64_t tc = 0; tc < g.nc; ++tc) {
+  for (int64_t tc = tc_begin; tc < tc_end; ++tc) {
     T* o = partial[omp_get_thread_num()].data();
     const int rt = g.nvc[tc] == 3 ? 0 : 1;
     for (int64_t sc = 0; sc < g.nc; ++sc) {
@@ -1319,7 +1324,7 @@ void assemble_cells_t(int kind, int op, double k, const CGrid& g, const CSpace& 
       }
       for (int a = 0; a < test.nb(tc); ++a)
         for (int b = 0; b < trial.nb(sc); ++b)
-          o[test.dofs[test.off[tc] + a] + ld * trial.dofs[trial.off[sc] + b]] += loc[a][b];
+          o[test.dofs[test.off[tc] + a] - row0 + ld * trial.dofs[trial.off[sc] + b]] += loc[a][b];
     }
   }
   for (int t = 0; t < nt; ++t)
@@ -1335,17 +1340,20 @@ extern "C" {
 // cells' vertex lists back to back (3 per triangle, 4 per quadrilateral in tensor order);
 // cell_types: 2 triangle, 3 quadrilateral. nq_tri / nq_quad: regular points per cell type.
 // part: 0 the whole operator, 1 its regular (non-adjacent) pairs, 2 its adjacent pairs, 3 the
-// adjacent pairs under the regular rule.
+// adjacent pairs under the regular rule. row1 > row0: rows row0..row1-1 only (P0 test space),
+// out (row1 - row0) x trial.
 int oracle_assemble_cells(int kind, int op, double k, const double* pts, int64_t nv, const int64_t* cell_vertices,
                           const uint8_t* cell_types, int64_t nc, int test_degree, int test_cont, int trial_degree,
-                          int trial_cont, int nq_tri, int nq_quad, int nsing, int nthreads, int part, void* out) {
+                          int trial_cont, int nq_tri, int nq_quad, int nsing, int nthreads, int part, int64_t row0,
+                          int64_t row1, void* out) {
   try {
     CGrid g(pts, nv, cell_vertices, cell_types, nc);
     CSpace ts(&g, test_degree, test_cont != 0), ss(&g, trial_degree, trial_cont != 0);
     if (kind == 0)
-      assemble_cells_t<double>(kind, op, k, g, ts, ss, nq_tri, nq_quad, nsing, nthreads, part, (double*)out);
+      assemble_cells_t<double>(kind, op, k, g, ts, ss, nq_tri, nq_quad, nsing, nthreads, part, row0, row1,
+                               (double*)out);
     else
-      assemble_cells_t<cplx>(kind, op, k, g, ts, ss, nq_tri, nq_quad, nsing, nthreads, part, (cplx*)out);
+      assemble_cells_t<cplx>(kind, op, k, g, ts, ss, nq_tri, nq_quad, nsing, nthreads, part, row0, row1, (cplx*)out);
     return 0;
   } catch (const std::exception& e) {
     g_err = e.what();

@@ -59,7 +59,7 @@ def lib():
         L.oracle_duffy_rule.argtypes = [i32, vp, vp, i32, vp, vp, vp]
         L.oracle_potential.argtypes = [i32, i32, dbl, vp, i64, vp, i64, i32, i32, i32, vp, i64, vp]
         L.oracle_assemble_cells.argtypes = [i32, i32, dbl, vp, i64, vp, vp, i64, i32, i32, i32, i32, i32, i32, i32,
-                                            i32, i32, vp]
+                                            i32, i32, i64, i64, vp]
         L.oracle_cells_space.argtypes = [vp, i64, vp, vp, i64, i32, i32, vp, vp]
         L.oracle_cell_pair_rule.argtypes = [vp, i64, vp, vp, i64, i64, i64, i32, vp, vp, vp, vp]
         _lib = L
@@ -209,20 +209,22 @@ def cells_space(pts, cells, degree, continuous):
 
 
 def assemble_cells(kind, op, pts, cells, test=(0, False), trial=(0, False), k=0.0, nq_tri=6, nq_quad=16, nsing=4,
-                   nthreads=0, part="all"):
+                   nthreads=0, part="all", rows=None):
     """Dense assembly on a grid of triangles and quadrilaterals (the general-cell oracle);
     part: "all", "regular" (non-adjacent pairs), "singular" (adjacent pairs) or "correction"
-    (adjacent pairs under the regular rule)."""
+    (adjacent pairs under the regular rule); rows=(r0, r1): those rows only (P0 test space)."""
     pts = np.ascontiguousarray(pts, dtype=np.float64)
     flat, types = _cells_arrays(cells)
     nt = cells_space(pts, cells, *test)[0]
     ns = cells_space(pts, cells, *trial)[0]
+    r0, r1 = rows if rows is not None else (0, 0)
     dtype = np.float64 if kind == "laplace" else np.complex128
-    out = np.zeros((nt, ns), dtype=dtype, order="F")
+    out = np.zeros((r1 - r0 if rows is not None else nt, ns), dtype=dtype, order="F")
     _check(lib().oracle_assemble_cells(KINDS[kind], OPS[op], float(k), _p(pts), len(pts), _p(flat), _p(types),
                                        len(types), test[0], int(test[1]), trial[0], int(trial[1]), nq_tri, nq_quad,
                                        nsing, nthreads or os.cpu_count() or 1,
-                                       {"all": 0, "regular": 1, "singular": 2, "correction": 3}[part], _p(out)))
+                                       {"all": 0, "regular": 1, "singular": 2, "correction": 3}[part], r0, r1,
+                                       _p(out)))
     return out
 
 

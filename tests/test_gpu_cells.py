@@ -260,3 +260,28 @@ def test_quadrilateral_device_overwrite_and_accumulate(sp, op):
     a.assemble_into_device(s, s, D, accumulate=True)
     torch.cuda.synchronize()
     assert np.linalg.norm(D.cpu().numpy() - 2 * R) / np.linalg.norm(R) <= 1e-12
+
+
+def test_quadrilateral_bench_size_rows():
+    """The measured quadrilateral workload (9 600-cell cubed sphere, 40 x 40 per face; Helmholtz
+    SL P0, k = 5, 4 x 4 Gauss points, 4-point Duffy rules) on the device: the first, a middle and
+    the last 64 rows against the oracle (the last rows hold mostly transposed stores of the
+    symmetric evaluation)."""
+    import torch
+
+    pts, cells = cubed_sphere(40)
+    g = Grid(pts, np.array(cells))
+    s = _space(g, "dp0")
+    a = _assembler("helmholtz", "single_layer", 5.0, _options(nq_quad=16, nsing=4))
+    D = a.assemble_device(s, s)
+    torch.cuda.synchronize()
+    n = len(cells)
+    for r0 in (0, n // 2, n - 64):
+        got = D[r0:r0 + 64].cpu().numpy()
+        ref = O.assemble_cells("helmholtz", "single_layer", pts, cells, k=5.0, nq_quad=16, nsing=4,
+                               rows=(r0, r0 + 64))
+        err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+        print(f"[cells] 9600 cells, rows {r0}..{r0 + 63}: rel err {err:.2e}", flush=True)
+        assert err <= 1e-12
+    del D
+    torch.cuda.empty_cache()

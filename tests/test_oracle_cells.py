@@ -210,3 +210,16 @@ def test_regular_and_correction_parts_against_numpy(mesh, kind, k, op):
             continue
         assert np.linalg.norm(C - adj) <= 1e-14 * np.linalg.norm(adj)
         assert np.linalg.norm(R + C - full) <= 1e-14 * np.linalg.norm(full)
+
+
+def test_row_range_equals_full_rows():
+    """The oracle's row range (P0 test spaces): rows r0..r1-1 of the full matrix, bit for bit;
+    a P1 test space is refused."""
+    pts, cells = _cube_mixed()
+    for kind, op, k in [("helmholtz", "single_layer", 2.0), ("laplace", "double_layer", 0.0)]:
+        A = O.assemble_cells(kind, op, pts, cells, k=k, trial=(1, True))
+        for r in [(0, 5), (10, 30), (len(cells) - 7, len(cells))]:
+            B = O.assemble_cells(kind, op, pts, cells, k=k, trial=(1, True), rows=r)
+            assert np.array_equal(B, A[r[0]:r[1]])
+    with pytest.raises(ValueError, match="P0 test space"):
+        O.assemble_cells("laplace", "single_layer", pts, cells, test=(1, True), rows=(0, 3))
