@@ -787,6 +787,7 @@ void cells_args(const Assembler& a, const FunctionSpace& trial, const FunctionSp
   A.k = a.k;
   A.ld = (long long)test.size;
   A.out_type = cells_out_type(a.dtype);
+  A.store_mode = dev::kAtomic;  // safe for any space; assemble_cells_device relaxes it when it can
   A.max_trial_points = sr.max_points;
   A.radius_sum = tt.max_radius + st.max_radius;
   // points of two cells sharing a vertex are at most 2 (rad_i + rad_j) apart: |k r| <= |k| 2 A.radius_sum
