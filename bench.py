@@ -29,18 +29,24 @@ FLOP_PER_PAIR_EPILOGUE = 4
 TILE_PROFILE = ROOT / "profiles" / "regular_p0_tiled_helmholtz_sl_p0_l5_k5.md"
 
 
-def profiled_traffic():
-    """(bytes per launch, source) from the committed counter summary, or (None, reason)."""
+def profiled_figure(key):
+    """The number on the committed counter summary's "key: value" line, or None."""
     import re
 
     try:
-        m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", TILE_PROFILE.read_text(), flags=re.M)
+        m = re.search(rf"^{key}:\s*([0-9.eE+]+)", TILE_PROFILE.read_text(), flags=re.M)
     except OSError:
         m = None
-    if not m:
+    return float(m.group(1)) if m else None
+
+
+def profiled_traffic():
+    """(bytes per launch, source) from the committed counter summary, or (None, reason)."""
+    v = profiled_figure("traffic_bytes_per_launch")
+    if v is None:
         return None, "no counter summary"
-    return float(m.group(1)), (f"{TILE_PROFILE.relative_to(ROOT)}: rocprofv3 --pmc FETCH_SIZE x 2.00 + WRITE_SIZE "
-                               "x 1.00 per dispatch (1 GiB copy calibration)")
+    return v, (f"{TILE_PROFILE.relative_to(ROOT)}: rocprofv3 --pmc FETCH_SIZE x 2.00 + WRITE_SIZE "
+               "x 1.00 per dispatch (1 GiB copy calibration)")
 
 
 def adjacent_ordered_pairs(cells, nv):
@@ -618,6 +624,8 @@ def main():
                          f"in {ct:.1f} s, oracle built {O.isa()}, OpenMP {threads} threads"}
 
     traffic, traffic_src = profiled_traffic()
+    on_profile = (args.level, args.k) == (5, 5.0)
+    hw_flop = profiled_figure("fp64_flop_per_launch") if on_profile else None
     line = {
         "metric": "assembled entries/s, Helmholtz SL P0 20480 tris: fp64 vs mixed @<=1e-6 rel err",
         "value": value, "unit": "entries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -639,8 +647,13 @@ def main():
         "kernels_ms": {"regular": float(np.mean(reg_ms)), "adjacent": float(np.mean(sing_ms))},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": traffic if (args.level, args.k) == (5, 5.0) else None,
-                     "traffic_source": traffic_src if (args.level, args.k) == (5, 5.0) else None,
+                     "traffic": traffic if on_profile else None,
+                     "traffic_source": traffic_src if on_profile else None,
+                     # the FP64 flops the hardware executed (SQ_INSTS_VALU_FLOPS_FP64 x 64 of one
+                     # dispatch, from the same summary) over this run's mean kernel time
+                     "counter_tflops": hw_flop / (np.mean(reg_ms) * 1e-3) / 1e12 if hw_flop else None,
+                     "counter_frac": hw_flop / (np.mean(reg_ms) * 1e-3) / 1e12 / FP64_PEAK_TFLOPS
+                     if hw_flop else None,
                      "hbm_write_GBps": write_gbps, "hbm_write_frac": write_gbps / HBM_PEAK_GBPS,
                      "basis": f"{n_reg} regular pairs x (36 x {FLOP_PER_QPP} + {FLOP_PER_PAIR_EPILOGUE}) nominal "
                               "flop per launch / mean regular-kernel HIP-event time"},
