@@ -285,3 +285,24 @@ def test_quadrilateral_bench_size_rows():
         assert err <= 1e-12
     del D
     torch.cuda.empty_cache()
+
+
+def test_batch_refuses_quadrilateral_grids():
+    """Batches are specialised to triangles: an item on a grid with quadrilateral cells is
+    refused when the batch is built, before any launch; a triangle item beside it is not
+    what fails."""
+    import torch
+
+    from bempp_amd.assembly.batch import BatchAssembler
+    from bempp_amd.shapes import icosphere
+
+    pts, quads = cube(1)
+    sq = _space(Grid(pts, np.array(quads)), "dp0")
+    st = _space(icosphere(1), "dp0")
+    a = _assembler("laplace", "single_layer", 0.0, _options())
+    with pytest.raises(ValueError, match="quadrilateral"):
+        BatchAssembler([a, a], [st, sq])
+    b = BatchAssembler([a], [st])  # the triangle item alone builds and assembles
+    b.assemble()
+    torch.cuda.synchronize()
+    assert np.isfinite(b.outputs[0].cpu().numpy()).all()
