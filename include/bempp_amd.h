@@ -313,7 +313,8 @@ int boundary_assembler_kernel_times(const BoundaryAssemblerT* assembler, float* 
  * n items, item i = assemblers[i] on the P0 space spaces[i] (test = trial), written to
  * the device buffer outputs[i] (n_i x n_i, Fortran order, the assembler's dtype, on the
  * current device). Items share kernel, operator, dtype, precision and regular rule;
- * wavenumbers, grids and singular rules may differ. All items run in one persistent
+ * wavenumbers, grids and singular rules may differ; every grid holds triangles only
+ * (an item on a grid with quadrilateral cells is refused). All items run in one persistent
  * launch (plus one launch for the adjacent pairs, and one for the near pairs in mixed
  * precision). The batch keeps device pointers into the grids' caches: free it before
  * its grids, spaces or assemblers; after grid_set_points on one of its grids,
